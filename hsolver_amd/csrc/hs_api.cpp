@@ -1,0 +1,668 @@
+// hs_api.cpp — C ABI of libhsolver_amd (declared in include/hsolver.h).
+//
+// Host-side sequencing of the reference's solver methods on one HIP stream.  Each
+// entry point corresponds to a reference member function (cited at its definition);
+// all arithmetic runs in the gfx950 kernels of hs_kernels.hip / hs_mg.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "hs_internal.h"
+
+int mask_fluid(hs_ctx* c, double* x);
+
+namespace {
+
+/* fluid_solver.h:1248-1261 cell widths with the reference's end fallbacks. */
+void axis_geom(const std::vector<double>& V, int n, double* pw, double* mw, double* plw) {
+    const long S = n + 1;
+    for (long i = 0; i < n + 3; ++i) { pw[i] = mw[i] = plw[i] = 0.0; }
+    for (long i = 1; i <= n + 1; ++i) {
+        const double c = (i < S) ? (V[i] - V[i - 1]) : (V[i - 1] - V[i - 2]);
+        const double m = (i - 2 >= 0) ? (V[i - 1] - V[i - 2]) : c;
+        const double p = (i + 1 < S) ? (V[i + 1] - V[i]) : c;
+        pw[i] = c; mw[i] = m; plw[i] = p;
+    }
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+    if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return HS_E_NOMEM;
+    if (hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return HS_E_HIP;
+    return HS_OK;
+}
+
+double* array_ptr(hs_ctx* c, int which) {
+    switch (which) {
+        case HS_A_U: return c->u; case HS_A_V: return c->v; case HS_A_W: return c->w;
+        case HS_A_UP: return c->up; case HS_A_VP: return c->vp; case HS_A_WP: return c->wp;
+        case HS_A_P: return c->p; case HS_A_PTEMP: return c->ptemp; case HS_A_PPRIME: return c->pprime;
+        case HS_A_TF: return c->tf; case HS_A_TS: return c->ts; case HS_A_Q: return c->q;
+        case HS_A_K: return c->k; case HS_A_B: return c->b;
+    }
+    return nullptr;
+}
+
+int64_t array_len(const hs_ctx* c, int which) {
+    const Dims& d = c->d;
+    switch (which) {
+        case HS_A_U: case HS_A_UP: return d.nU;
+        case HS_A_V: case HS_A_VP: return d.nV;
+        case HS_A_W: case HS_A_WP: return d.nW;
+        case HS_A_P: case HS_A_Q: case HS_A_K: return d.nC;
+        case HS_A_PTEMP: case HS_A_PPRIME: case HS_A_TF: case HS_A_TS: case HS_A_B: return d.nP;
+    }
+    return -1;
+}
+
+// connected solid regions (6-connectivity, mesh.cpp:298-454); bbox of the first
+int solid_regions(hs_ctx* c) {
+    const Dims& d = c->d;
+    std::vector<int> lab(d.nC, -1);
+    std::vector<long> stack;
+    int nreg = 0;
+    for (long c0 = 0; c0 < d.nC; ++c0) {
+        if (c->type[c0] != 1 || lab[c0] >= 0) continue;
+        int bb[6] = {1 << 30, -1, 1 << 30, -1, 1 << 30, -1};
+        stack.assign(1, c0);
+        lab[c0] = nreg;
+        while (!stack.empty()) {
+            const long q = stack.back();
+            stack.pop_back();
+            const long i = q % d.nx, j = (q / d.nx) % d.ny, k = q / ((long)d.nx * d.ny);
+            bb[0] = std::min<long>(bb[0], i + 1); bb[1] = std::max<long>(bb[1], i + 1);
+            bb[2] = std::min<long>(bb[2], j + 1); bb[3] = std::max<long>(bb[3], j + 1);
+            bb[4] = std::min<long>(bb[4], k + 1); bb[5] = std::max<long>(bb[5], k + 1);
+            const long nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            for (auto& t : nb) {
+                if (t[0] < 0 || t[0] >= d.nx || t[1] < 0 || t[1] >= d.ny || t[2] < 0 || t[2] >= d.nz) continue;
+                const long cn = idxC(d, t[0], t[1], t[2]);
+                if (c->type[cn] == 1 && lab[cn] < 0) { lab[cn] = nreg; stack.push_back(cn); }
+            }
+        }
+        if (nreg == 0) std::memcpy(c->sb, bb, sizeof bb);
+        ++nreg;
+    }
+    return nreg;
+}
+
+// interface face pairs in the reference's list order: x faces (k,j,i), y faces, z faces
+// (fluid_solver.h:900-1229 / mobs_solver.h:121-268).
+void interface_pairs(const hs_ctx* c, std::vector<long>& F, std::vector<long>& S) {
+    const Dims& d = c->d;
+    auto fl = [&](long i, long j, long k) { return c->type[idxC(d, i - 1, j - 1, k - 1)] == 0; };
+    for (long k = 1; k <= d.nz; ++k)
+        for (long j = 1; j <= d.ny; ++j)
+            for (long i = 2; i <= d.nx; ++i) {
+                const bool fa = fl(i - 1, j, k), fb = fl(i, j, k);
+                if (fa && !fb) { F.push_back(idxP(d, i - 1, j, k)); S.push_back(idxP(d, i, j, k)); }
+                else if (!fa && fb) { F.push_back(idxP(d, i, j, k)); S.push_back(idxP(d, i - 1, j, k)); }
+            }
+    for (long k = 1; k <= d.nz; ++k)
+        for (long j = 2; j <= d.ny; ++j)
+            for (long i = 1; i <= d.nx; ++i) {
+                const bool fa = fl(i, j - 1, k), fb = fl(i, j, k);
+                if (fa && !fb) { F.push_back(idxP(d, i, j - 1, k)); S.push_back(idxP(d, i, j, k)); }
+                else if (!fa && fb) { F.push_back(idxP(d, i, j, k)); S.push_back(idxP(d, i, j - 1, k)); }
+            }
+    for (long k = 2; k <= d.nz; ++k)
+        for (long j = 1; j <= d.ny; ++j)
+            for (long i = 1; i <= d.nx; ++i) {
+                const bool fa = fl(i, j, k - 1), fb = fl(i, j, k);
+                if (fa && !fb) { F.push_back(idxP(d, i, j, k - 1)); S.push_back(idxP(d, i, j, k)); }
+                else if (!fa && fb) { F.push_back(idxP(d, i, j, k)); S.push_back(idxP(d, i, j, k - 1)); }
+            }
+}
+
+double count_solved(const hs_ctx* c, int comp) {
+    const Dims& d = c->d;
+    auto fl = [&](long i, long j, long k) {
+        return i >= 1 && i <= d.nx && j >= 1 && j <= d.ny && k >= 1 && k <= d.nz && c->type[idxC(d, i - 1, j - 1, k - 1)] == 0;
+    };
+    double n = 0;
+    for (long k = 1; k <= d.nz + (comp == 2); ++k)
+        for (long j = 1; j <= d.ny + (comp == 1); ++j)
+            for (long i = 1; i <= d.nx + (comp == 0); ++i) {
+                const bool a = comp == 0 ? fl(i - 1, j, k) : (comp == 1 ? fl(i, j - 1, k) : fl(i, j, k - 1));
+                n += (a && fl(i, j, k)) ? 1 : 0;
+            }
+    return n;
+}
+
+bool modeR(const hs_ctx* c) { return c->prm.mode == HS_MODE_REF; }
+
+// read the sweep control block back (one synchronisation)
+int read_wctl(hs_ctx* c) {
+    HS_CHECK(hipMemcpyAsync(c->h_wctl, c->wctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int apply_bc_comp(hs_ctx* c, int comp) {
+    if (modeR(c)) return comp == 0 ? bc_x_ref(c) : (comp == 1 ? bc_y_ref(c) : bc_z_ref(c));
+    return bc_cfg_outflow(c, comp);
+}
+
+// Runs a Gauss-Seidel loop whose stop rule lives in wctl.  The device gates every sweep
+// on wctl->active, so when the tolerance is 0 (fixed sweep count) all sweeps are queued
+// without a host round trip; otherwise the host checks after each sweep.
+template <typename SweepFn, typename AfterFn>
+int gs_loop(hs_ctx* c, double n, double tol, int maxsteps, int quirk, SweepFn sweep, AfterFn after,
+            int* sweeps, double* resid) {
+    HS_TRY(sweep_begin(c, n, tol, maxsteps, quirk));
+    const bool fixed = !(tol > 0.0) && quirk == 0;
+    for (int s = 0; s < maxsteps; ++s) {
+        HS_TRY(sweep());
+        HS_TRY(after());
+        if (!fixed) {
+            HS_TRY(read_wctl(c));
+            if (!c->h_wctl->active) break;
+        }
+    }
+    HS_TRY(read_wctl(c));
+    if (sweeps) *sweeps = c->h_wctl->step;
+    if (resid) *resid = c->h_wctl->global;
+    if (std::isnan(c->h_wctl->global)) return HS_E_NAN;
+    return HS_OK;
+}
+
+int do_momentum(hs_ctx* c, int comp, int* sweeps, double* resid) {
+    const Dims& d = c->d;
+    double n;
+    int quirk;
+    if (modeR(c)) {
+        const double cx = d.nx, cy = d.ny, cz = d.nz;
+        n = comp == 0 ? cz * cy * (cx + 1) : (comp == 1 ? cz * (cy + 1) * cx : (cz + 1) * cy * cx);
+        quirk = 1;
+    } else {
+        n = std::max(1.0, c->n_solved[comp]);
+        quirk = 0;
+    }
+    return gs_loop(
+        c, n, c->prm.mom_tol, c->prm.mom_sweeps_max, quirk, [&] { return momentum_sweep(c, comp); },
+        [&] { return apply_bc_comp(c, comp); }, sweeps, resid);
+}
+
+int do_energy(hs_ctx* c, int* sweeps, double* resid) {
+    const Dims& d = c->d;
+    if (modeR(c))
+        return gs_loop(c, (double)d.nx * d.ny * d.nz, c->prm.t_tol, c->prm.t_sweeps_max, 1,
+                       [&] { return energy_sweep(c); }, [&] { return HS_OK; }, sweeps, resid);
+    return gs_loop(c, std::max(1.0, c->n_tcells), c->prm.t_tol, c->prm.t_sweeps_max, 0,
+                   [&] { return energy_sweep(c); }, [&] { return bc_cfg_outflow(c, 3); }, sweeps, resid);
+}
+
+int do_pressure(hs_ctx* c, hs_stats* st) {
+    if (!c->have_matrix || !(modeR(c) && c->prm.freeze_p_matrix)) HS_TRY(assemble_matrix(c));
+    HS_TRY(assemble_rhs(c));
+    hs_stats loc{};
+    int e;
+    if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc);
+    else e = cg_solve(c, c->b, c->kx, &loc);
+    if (e != HS_OK) return e;
+    HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
+    double rel, bn;
+    HS_TRY(true_relres(c, c->b, c->kx, &rel, &bn));
+    if (st) {
+        st->p_iters = loc.p_iters; st->p_converged = loc.p_converged; st->p_relres = loc.p_relres;
+        st->p_true_relres = rel; st->mass_imbalance = bn;
+    }
+    if (c->prm.fail_on_noconv && !loc.p_converged) return HS_E_NOCONV;
+    return HS_OK;
+}
+
+int do_correct(hs_ctx* c) {
+    HS_TRY(correct_velocity(c));
+    HS_TRY(update_pressure(c));
+    if (!modeR(c)) HS_TRY(bc_cfg_outflow(c, 0));
+    return HS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hs_params_default(int32_t mode, hs_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->mode = mode;
+    p->order = HS_ORDER_LEX;
+    p->rho = 1.0;
+    p->nu = 0.001;
+    p->alpha_f = 0.001;
+    p->k_ratio = 1.0;
+    p->u_lid = 0.1;
+    p->relax_u = p->relax_p = p->relax_t = p->relax_ts = 0.3;
+    p->mom_tol = 2e-3;
+    p->t_tol = 2e-3;
+    p->ts_tol = 1e-3;
+    p->mom_sweeps_max = p->t_sweeps_max = p->ts_sweeps_max = 100;
+    p->p_tol = 1e-5;
+    p->p_maxit = 1000;
+    p->freeze_p_matrix = 1;
+    p->precond = HS_PRECOND_NONE;
+    p->precision = HS_PREC_FP64;
+    p->mg_levels = 4;
+    p->mg_pre = 2;
+    p->mg_post = 2;
+    p->mg_coarse_sweeps = 40;
+    p->mg_dense_coarse = 0;
+    p->t_fluid_init = 293.0;
+    p->t_solid_init = 333.0;
+    p->t_wall = 293.0;
+    if (mode == HS_MODE_CONFIG) {
+        p->freeze_p_matrix = 0;
+        p->relax_u = 0.7;
+        p->relax_p = 0.3;
+        p->relax_t = 1.0;
+        p->relax_ts = 1.0;
+        p->mom_sweeps_max = 1;
+        p->t_sweeps_max = 1;
+        p->ts_sweeps_max = 0;
+        p->mom_tol = 0.0;
+        p->t_tol = 0.0;
+        p->p_tol = 1e-8;
+        p->outflow = 1;
+        p->u_lid = 0.0;
+        p->u_in = 1.0;
+    }
+}
+
+const char* hs_strerror(int code) {
+    switch (code) {
+        case HS_OK: return "ok";
+        case HS_E_ARG: return "invalid argument or unsupported grid";
+        case HS_E_HIP: return "HIP runtime error";
+        case HS_E_NOCONV: return "pressure-correction solve did not converge";
+        case HS_E_NAN: return "NaN or Inf in a residual";
+        case HS_E_IO: return "file could not be read";
+        case HS_E_STATE: return "call out of order";
+        case HS_E_NOMEM: return "device allocation failed";
+    }
+    return "unknown error";
+}
+
+int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
+    if (!g || !p || !out || g->nx < 3 || g->ny < 3 || g->nz < 3 || !g->xv || !g->yv || !g->zv || !g->cell_type)
+        return HS_E_ARG;
+    if (p->mode != HS_MODE_REF && p->mode != HS_MODE_CONFIG) return HS_E_ARG;
+    hs_ctx* c = new (std::nothrow) hs_ctx();
+    if (!c) return HS_E_NOMEM;
+    c->prm = *p;
+    c->d = make_dims(g->nx, g->ny, g->nz);
+    const Dims& d = c->d;
+    c->X.assign(g->xv, g->xv + d.nx + 1);
+    c->Y.assign(g->yv, g->yv + d.ny + 1);
+    c->Z.assign(g->zv, g->zv + d.nz + 1);
+    c->type.assign(g->cell_type, g->cell_type + d.nC);
+    for (auto t : c->type) if (t > 1) { delete c; return HS_E_ARG; }
+    c->solid_regions = solid_regions(c);
+    if (modeR(c)) {
+        // one fluid region spanning the domain, at most one solid region (Q16)
+        if (c->solid_regions > 1) { delete c; return HS_E_ARG; }
+        bool any[6] = {false, false, false, false, false, false};
+        for (long k = 0; k < d.nz; ++k)
+            for (long j = 0; j < d.ny; ++j)
+                for (long i = 0; i < d.nx; ++i)
+                    if (c->type[idxC(d, i, j, k)] == 0) {
+                        any[0] |= i == 0; any[1] |= i == d.nx - 1; any[2] |= j == 0;
+                        any[3] |= j == d.ny - 1; any[4] |= k == 0; any[5] |= k == d.nz - 1;
+                    }
+        for (bool a : any) if (!a) { delete c; return HS_E_ARG; }
+    } else {
+        for (int comp = 0; comp < 3; ++comp) c->n_solved[comp] = count_solved(c, comp);
+        c->n_tcells = (double)d.nC;
+    }
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HS_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return HS_E_HIP; }
+
+    int e = HS_OK;
+    auto A = [&](double** ptr, long n) { if (e == HS_OK) e = dalloc(ptr, (size_t)n); };
+    const long gl = std::max(d.nx, std::max(d.ny, d.nz)) + 3;
+    A(&c->geo, 9 * gl);
+    A(&c->u, d.nU); A(&c->up, d.nU); A(&c->v, d.nV); A(&c->vp, d.nV); A(&c->w, d.nW); A(&c->wp, d.nW);
+    A(&c->p, d.nC); A(&c->psnap, d.nC); A(&c->q, d.nC); A(&c->k, d.nC);
+    A(&c->ptemp, d.nP); A(&c->pprime, d.nP); A(&c->tf, d.nP); A(&c->ts, d.nP); A(&c->b, d.nP);
+    if (modeR(c)) { A(&c->ax, d.nP); A(&c->ay, d.nP); A(&c->az, d.nP); }
+    else { A(&c->ax, d.nU); A(&c->ay, d.nV); A(&c->az, d.nW); }
+    A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
+    A(&c->dres, std::max(std::max(d.nU, d.nV), std::max(d.nW, d.nP)));
+    A(&c->partials, 65536);
+    if (e == HS_OK) e = dalloc(&c->counter, 4);
+    if (e == HS_OK) e = dalloc(&c->cls, (size_t)d.nP);
+    if (e == HS_OK) e = dalloc(&c->sctl, 1);
+    if (e == HS_OK) e = dalloc(&c->wctl, 1);
+    if (e == HS_OK && hipHostMalloc((void**)&c->h_sctl, sizeof(SolveCtl), 0) != hipSuccess) e = HS_E_NOMEM;
+    if (e == HS_OK && hipHostMalloc((void**)&c->h_wctl, sizeof(SweepCtl), 0) != hipSuccess) e = HS_E_NOMEM;
+    if (e != HS_OK) { hs_destroy(c); return e; }
+
+    // geometry
+    std::vector<double> geo(9 * gl, 0.0);
+    axis_geom(c->X, d.nx, &geo[0 * gl], &geo[1 * gl], &geo[2 * gl]);
+    axis_geom(c->Y, d.ny, &geo[3 * gl], &geo[4 * gl], &geo[5 * gl]);
+    axis_geom(c->Z, d.nz, &geo[6 * gl], &geo[7 * gl], &geo[8 * gl]);
+    const int gl_n[9] = {d.nx, d.nx, d.nx, d.ny, d.ny, d.ny, d.nz, d.nz, d.nz};
+    (void)gl_n;
+    if (hipMemcpy(c->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { hs_destroy(c); return HS_E_HIP; }
+    c->g.xp = c->geo + 0 * gl; c->g.xw = c->geo + 1 * gl; c->g.xe = c->geo + 2 * gl;
+    c->g.yp = c->geo + 3 * gl; c->g.ys = c->geo + 4 * gl; c->g.yn = c->geo + 5 * gl;
+    c->g.zp = c->geo + 6 * gl; c->g.zf = c->geo + 7 * gl; c->g.zb = c->geo + 8 * gl;
+
+    // cell classes
+    std::vector<uint8_t> cls(d.nP, CELL_GHOST);
+    for (long k = 1; k <= d.nz; ++k)
+        for (long j = 1; j <= d.ny; ++j)
+            for (long i = 1; i <= d.nx; ++i)
+                cls[idxP(d, i, j, k)] = c->type[idxC(d, i - 1, j - 1, k - 1)] == 0 ? CELL_FLUID : CELL_SOLID;
+    if (hipMemcpy(c->cls, cls.data(), d.nP, hipMemcpyHostToDevice) != hipSuccess) { hs_destroy(c); return HS_E_HIP; }
+
+    // interface pairs
+    std::vector<long> F, S;
+    interface_pairs(c, F, S);
+    c->n_if = (long)F.size();
+    if (c->n_if > 0) {
+        if (dalloc(&c->if_fluid, F.size()) != HS_OK || dalloc(&c->if_solid, S.size()) != HS_OK) { hs_destroy(c); return HS_E_NOMEM; }
+        hipMemcpy(c->if_fluid, F.data(), F.size() * sizeof(long), hipMemcpyHostToDevice);
+        hipMemcpy(c->if_solid, S.data(), S.size() * sizeof(long), hipMemcpyHostToDevice);
+    }
+
+    // initial fills (fluid_solver.h:98-108, mobs_solver.h:55)
+    std::vector<double> kk(d.nC);
+    for (long i = 0; i < d.nC; ++i) kk[i] = c->type[i] == 1 ? p->alpha_f * p->k_ratio : p->alpha_f;
+    if (hipMemcpy(c->k, kk.data(), d.nC * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { hs_destroy(c); return HS_E_HIP; }
+    if (modeR(c)) {
+        e = launch_fill(c, c->tf, d.nP, p->t_fluid_init);
+    } else {
+        std::vector<double> t(d.nP, p->t_fluid_init);
+        for (long k = 0; k < d.nz + 2; ++k)
+            for (long j = 0; j < d.ny + 2; ++j)
+                for (long i = 0; i < d.nx + 2; ++i)
+                    if (i == 0 || j == 0 || k == 0 || i == d.nx + 1 || j == d.ny + 1 || k == d.nz + 1)
+                        t[idxP(d, i, j, k)] = p->t_wall;
+        if (hipMemcpy(c->tf, t.data(), d.nP * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { hs_destroy(c); return HS_E_HIP; }
+    }
+    if (e == HS_OK) e = launch_fill(c, c->ts, d.nP, p->t_solid_init);
+    if (e == HS_OK && hipStreamSynchronize(c->stream) != hipSuccess) e = HS_E_HIP;
+    if (e != HS_OK) { hs_destroy(c); return e; }
+    *out = c;
+    return HS_OK;
+}
+
+void hs_destroy(hs_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    gmg_free(c);
+    double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
+                      c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->ks, c->kx, c->kz,
+                      c->kq, c->dres, c->partials};
+    for (double* b : bufs) if (b) hipFree(b);
+    if (c->counter) hipFree(c->counter);
+    if (c->cls) hipFree(c->cls);
+    if (c->sctl) hipFree(c->sctl);
+    if (c->wctl) hipFree(c->wctl);
+    if (c->if_fluid) hipFree(c->if_fluid);
+    if (c->if_solid) hipFree(c->if_solid);
+    if (c->h_sctl) hipHostFree(c->h_sctl);
+    if (c->h_wctl) hipHostFree(c->h_wctl);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int64_t hs_array_size(const hs_ctx* c, int32_t which) { return c ? array_len(c, which) : -1; }
+
+/* FluidSolver::init fluid_solver.h:882-1237 + MobSolver::init mobs_solver.h:116-288. */
+int hs_init(hs_ctx* c) {
+    if (!c) return HS_E_ARG;
+    if (modeR(c)) {
+        HS_TRY(bc_x_ref(c)); HS_TRY(bc_y_ref(c)); HS_TRY(bc_z_ref(c));
+        HS_TRY(cell_bc_ref(c));
+        HS_TRY(zero_interface_faces(c));
+    } else {
+        HS_TRY(bc_cfg_init(c));
+    }
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    c->inited = true;
+    return HS_OK;
+}
+
+/* solveXmomentum :577 / solveYmomentum :491 / solveZmomentum :407. */
+int hs_momentum(hs_ctx* c, int32_t comp, int32_t* sweeps, double* residual) {
+    if (!c || comp < 0 || comp > 2) return HS_E_ARG;
+    int s = 0; double r = 0;
+    HS_TRY(do_momentum(c, comp, &s, &r));
+    if (sweeps) *sweeps = s;
+    if (residual) *residual = r;
+    return HS_OK;
+}
+
+int hs_pressure_correction(hs_ctx* c, hs_stats* st) {
+    if (!c) return HS_E_ARG;
+    return do_pressure(c, st);
+}
+
+int hs_correct(hs_ctx* c) {
+    if (!c) return HS_E_ARG;
+    HS_TRY(do_correct(c));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_fluid_energy(hs_ctx* c, int32_t* sweeps, double* residual) {
+    if (!c) return HS_E_ARG;
+    int s = 0; double r = 0;
+    HS_TRY(do_energy(c, &s, &r));
+    if (sweeps) *sweeps = s;
+    if (residual) *residual = r;
+    return HS_OK;
+}
+
+/* FluidSolver::solve fluid_solver.h:1401-1633. */
+int hs_fluid_solve(hs_ctx* c, hs_stats* st) {
+    if (!c) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    hs_stats loc{};
+    if (st) { loc.ts_sweeps = st->ts_sweeps; loc.ts_residual = st->ts_residual; }
+    for (int comp = 0; comp < 3; ++comp) HS_TRY(do_momentum(c, comp, &loc.mom_sweeps[comp], &loc.mom_residual[comp]));
+    HS_TRY(do_pressure(c, &loc));
+    HS_TRY(do_correct(c));
+    HS_TRY(do_energy(c, &loc.t_sweeps, &loc.t_residual));
+    if (!modeR(c)) {
+        double heat = 0, res = 0;
+        HS_TRY(energy_balance(c, &heat, &res));
+        loc.heat_in = heat;
+        loc.energy_residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
+    }
+    if (st) *st = loc;
+    return HS_OK;
+}
+
+/* MobSolver::solve mobs_solver.h:303-371. */
+int hs_solid_solve(hs_ctx* c, hs_stats* st) {
+    if (!c) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    int s = 0; double r = 0;
+    if (modeR(c) && c->solid_regions > 0) {
+        const double n = (double)(c->sb[1] - c->sb[0] + 1) * (c->sb[3] - c->sb[2] + 1) * (c->sb[5] - c->sb[4] + 1);
+        HS_TRY(gs_loop(c, n, c->prm.ts_tol, c->prm.ts_sweeps_max, 2, [&] { return solid_sweep(c); },
+                       [&] { return HS_OK; }, &s, &r));
+    }
+    if (st) { st->ts_sweeps = s; st->ts_residual = r; }
+    return HS_OK;
+}
+
+/* Solver::solve exchange block solver.h:62-148. */
+int hs_exchange(hs_ctx* c) {
+    if (!c) return HS_E_ARG;
+    if (modeR(c)) HS_TRY(exchange_kernel(c));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+/* FluidSolver::setForNext fluid_solver.h:1634-1682 + MobSolver::setForNext. */
+int hs_set_for_next(hs_ctx* c) {
+    if (!c) return HS_E_ARG;
+    if (modeR(c)) {
+        HS_TRY(bc_x_ref(c)); HS_TRY(bc_y_ref(c)); HS_TRY(bc_z_ref(c));
+        HS_TRY(cell_bc_ref(c));
+        HS_TRY(ptemp_from_p_all(c));
+    } else {
+        for (int comp = 0; comp < 4; ++comp) HS_TRY(bc_cfg_outflow(c, comp));
+    }
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_step(hs_ctx* c, hs_stats* st) {
+    hs_stats loc{};
+    HS_TRY(hs_fluid_solve(c, &loc));
+    HS_TRY(hs_solid_solve(c, &loc));
+    HS_TRY(hs_exchange(c));
+    HS_TRY(hs_set_for_next(c));
+    if (st) *st = loc;
+    return HS_OK;
+}
+
+int hs_run(hs_ctx* c, int32_t n, hs_stats* st) {
+    for (int32_t it = 0; it < n; ++it) HS_TRY(hs_step(c, st ? st + it : nullptr));
+    return HS_OK;
+}
+
+int hs_pressure_apply(hs_ctx* c, const double* x_host, double* y_host) {
+    if (!c || !x_host || !y_host) return HS_E_ARG;
+    if (!c->have_matrix) return HS_E_STATE;
+    const long n = c->d.nP;
+    HS_CHECK(hipMemcpyAsync(c->kz, x_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_TRY(mask_fluid(c, c->kz));
+    HS_TRY(spmv(c, c->kz, c->kq));
+    HS_CHECK(hipMemcpyAsync(y_host, c->kq, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_pressure_solve(hs_ctx* c, const double* b_host, double* x_host, hs_stats* st) {
+    if (!c || !b_host || !x_host) return HS_E_ARG;
+    if (!c->have_matrix) return HS_E_STATE;
+    const long n = c->d.nP;
+    HS_CHECK(hipMemcpyAsync(c->kz, b_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_TRY(mask_fluid(c, c->kz));
+    hs_stats loc{};
+    int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kz, c->kx, &loc) : cg_solve(c, c->kz, c->kx, &loc);
+    if (e != HS_OK) return e;
+    double rel, bn;
+    HS_TRY(true_relres(c, c->kz, c->kx, &rel, &bn));
+    loc.p_true_relres = rel;
+    loc.mass_imbalance = bn;
+    HS_CHECK(hipMemcpyAsync(x_host, c->kx, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    if (st) *st = loc;
+    return HS_OK;
+}
+
+int hs_get_array(hs_ctx* c, int32_t which, double* host, int64_t n) {
+    if (!c || !host) return HS_E_ARG;
+    const int64_t s = array_len(c, which);
+    if (s < 0 || n != s) return HS_E_ARG;
+    HS_CHECK(hipMemcpyAsync(host, array_ptr(c, which), s * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_put_array(hs_ctx* c, int32_t which, const double* host, int64_t n) {
+    if (!c || !host) return HS_E_ARG;
+    const int64_t s = array_len(c, which);
+    if (s < 0 || n != s) return HS_E_ARG;
+    HS_CHECK(hipMemcpyAsync(array_ptr(c, which), host, s * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_get_fields(hs_ctx* c, double* u, double* v, double* w, double* p, double* t) {
+    if (!c) return HS_E_ARG;
+    const Dims& d = c->d;
+    std::vector<double> U(d.nU), V(d.nV), W(d.nW), P(d.nC), TF(d.nP), TS(d.nP);
+    HS_TRY(hs_get_array(c, HS_A_U, U.data(), d.nU));
+    HS_TRY(hs_get_array(c, HS_A_V, V.data(), d.nV));
+    HS_TRY(hs_get_array(c, HS_A_W, W.data(), d.nW));
+    HS_TRY(hs_get_array(c, HS_A_P, P.data(), d.nC));
+    HS_TRY(hs_get_array(c, HS_A_TF, TF.data(), d.nP));
+    HS_TRY(hs_get_array(c, HS_A_TS, TS.data(), d.nP));
+    const long nx = d.nx, ny = d.ny, nz = d.nz;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j) {
+            for (long i = 0; i <= nx; ++i) if (u) u[i + (nx + 1) * (j + ny * k)] = U[idxU(d, i + 1, j + 1, k + 1)];
+            for (long i = 0; i < nx; ++i) {
+                const long cc = idxC(d, i, j, k);
+                if (p) p[cc] = P[cc];
+                if (t) {
+                    const long c2 = idxP(d, i + 1, j + 1, k + 1);
+                    t[cc] = (modeR(c) && c->type[cc] == 1) ? TS[c2] : TF[c2];
+                }
+            }
+        }
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j <= ny; ++j)
+            for (long i = 0; i < nx; ++i) if (v) v[i + nx * (j + (ny + 1) * k)] = V[idxV(d, i + 1, j + 1, k + 1)];
+    for (long k = 0; k <= nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) if (w) w[i + nx * (j + ny * k)] = W[idxW(d, i + 1, j + 1, k + 1)];
+    return HS_OK;
+}
+
+int hs_put_fields(hs_ctx* c, const double* u, const double* v, const double* w, const double* p, const double* t) {
+    if (!c) return HS_E_ARG;
+    const Dims& d = c->d;
+    std::vector<double> U(d.nU), V(d.nV), W(d.nW), P(d.nC), TF(d.nP), TS(d.nP), PT(d.nP);
+    HS_TRY(hs_get_array(c, HS_A_U, U.data(), d.nU));
+    HS_TRY(hs_get_array(c, HS_A_V, V.data(), d.nV));
+    HS_TRY(hs_get_array(c, HS_A_W, W.data(), d.nW));
+    HS_TRY(hs_get_array(c, HS_A_P, P.data(), d.nC));
+    HS_TRY(hs_get_array(c, HS_A_TF, TF.data(), d.nP));
+    HS_TRY(hs_get_array(c, HS_A_TS, TS.data(), d.nP));
+    HS_TRY(hs_get_array(c, HS_A_PTEMP, PT.data(), d.nP));
+    const long nx = d.nx, ny = d.ny, nz = d.nz;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j) {
+            for (long i = 0; i <= nx; ++i) if (u) U[idxU(d, i + 1, j + 1, k + 1)] = u[i + (nx + 1) * (j + ny * k)];
+            for (long i = 0; i < nx; ++i) {
+                const long cc = idxC(d, i, j, k), c2 = idxP(d, i + 1, j + 1, k + 1);
+                if (p) { P[cc] = p[cc]; PT[c2] = p[cc]; }
+                if (t) { if (modeR(c) && c->type[cc] == 1) TS[c2] = t[cc]; else TF[c2] = t[cc]; }
+            }
+        }
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j <= ny; ++j)
+            for (long i = 0; i < nx; ++i) if (v) V[idxV(d, i + 1, j + 1, k + 1)] = v[i + nx * (j + (ny + 1) * k)];
+    for (long k = 0; k <= nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) if (w) W[idxW(d, i + 1, j + 1, k + 1)] = w[i + nx * (j + ny * k)];
+    HS_TRY(hs_put_array(c, HS_A_U, U.data(), d.nU));
+    HS_TRY(hs_put_array(c, HS_A_V, V.data(), d.nV));
+    HS_TRY(hs_put_array(c, HS_A_W, W.data(), d.nW));
+    HS_TRY(hs_put_array(c, HS_A_P, P.data(), d.nC));
+    HS_TRY(hs_put_array(c, HS_A_PTEMP, PT.data(), d.nP));
+    HS_TRY(hs_put_array(c, HS_A_TF, TF.data(), d.nP));
+    HS_TRY(hs_put_array(c, HS_A_TS, TS.data(), d.nP));
+    return HS_OK;
+}
+
+int hs_set_source(hs_ctx* c, const double* q) {
+    if (!c || !q) return HS_E_ARG;
+    return hs_put_array(c, HS_A_Q, q, c->d.nC);
+}
+
+int hs_set_conductivity(hs_ctx* c, const double* k) {
+    if (!c || !k) return HS_E_ARG;
+    return hs_put_array(c, HS_A_K, k, c->d.nC);
+}
+
+/* applyCellBoundaryCondition fluid_solver.h:324-405 on its own (tests / host drivers). */
+int hs_cell_bc(hs_ctx* c) {
+    if (!c) return HS_E_ARG;
+    if (!modeR(c)) return HS_E_STATE;
+    HS_TRY(cell_bc_ref(c));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// hs_internal.h — device context and helpers shared by the gfx950 kernels and the
+// C-ABI host code of libhsolver_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/hsolver.h"
+
+#define HS_CHECK(expr)                                                              \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess) {                                                     \
+            fprintf(stderr, "hsolver: HIP error %s at %s:%d\n", hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                            \
+            return HS_E_HIP;                                                        \
+        }                                                                           \
+    } while (0)
+
+#define HS_TRY(expr)                  \
+    do {                              \
+        int _r = (expr);              \
+        if (_r != HS_OK) return _r;   \
+    } while (0)
+
+// Grid extents and strides for the padded layouts (x fastest).
+struct Dims {
+    int nx, ny, nz;
+    // P layout (cells): (nx+2)(ny+2)(nz+2); W layout shares the i,j strides with P.
+    long sPy, sPz;
+    long sUy, sUz;   // U layout (nx+3)(ny+2)(nz+2)
+    long sVy, sVz;   // V layout (nx+2)(ny+3)(nz+2)
+    long nP, nU, nV, nW, nC;
+};
+
+inline Dims make_dims(int nx, int ny, int nz) {
+    Dims d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.sPy = nx + 2; d.sPz = (long)(nx + 2) * (ny + 2);
+    d.sUy = nx + 3; d.sUz = (long)(nx + 3) * (ny + 2);
+    d.sVy = nx + 2; d.sVz = (long)(nx + 2) * (ny + 3);
+    d.nP = d.sPz * (nz + 2);
+    d.nU = d.sUz * (nz + 2);
+    d.nV = d.sVz * (nz + 2);
+    d.nW = d.sPz * (nz + 3);
+    d.nC = (long)nx * ny * nz;
+    return d;
+}
+
+__host__ __device__ inline long idxP(const Dims& d, long i, long j, long k) { return i + d.sPy * j + d.sPz * k; }
+__host__ __device__ inline long idxU(const Dims& d, long i, long j, long k) { return i + d.sUy * j + d.sUz * k; }
+__host__ __device__ inline long idxV(const Dims& d, long i, long j, long k) { return i + d.sVy * j + d.sVz * k; }
+__host__ __device__ inline long idxW(const Dims& d, long i, long j, long k) { return i + d.sPy * j + d.sPz * k; }
+__host__ __device__ inline long idxC(const Dims& d, long i, long j, long k) { return i + (long)d.nx * (j + (long)d.ny * k); }
+
+// Cell classes stored per padded cell (P layout).
+enum : uint8_t { CELL_GHOST = 0, CELL_FLUID = 1, CELL_SOLID = 2 };
+
+// 1D geometry per padded index: width of the cell, of the minus neighbour, of the plus
+// neighbour (the reference's xp/xw/xe, yp/ys/yn, zp/zf/zb with its end fallbacks).
+struct Geo {
+    const double *xp, *xw, *xe, *yp, *ys, *yn, *zp, *zf, *zb;
+};
+
+// Scalar state of one Krylov solve or one sweep loop, resident in device memory so the
+// control decisions are taken on the GPU without a host round trip per iteration.
+struct SolveCtl {
+    double bb;         // <b,b>
+    double gamma;      // <r,z>
+    double gamma_old;
+    double alpha;
+    double beta;
+    double rr;         // <r,r> at the last update
+    double sdotp;
+    double eps;        // tol^2
+    int    iters;
+    int    done;       // 1: stop launching (converged / breakdown / maxit)
+    int    converged;
+    int    maxit;
+    double red[8];     // generic reduction results
+};
+
+struct SweepCtl {
+    double residual;   // reference accumulator (Mode R) or last sweep value
+    double global;     // sqrt(...) compared with the tolerance
+    double ncount;     // normaliser N
+    double tol;
+    int    step;
+    int    maxsteps;
+    int    active;     // 1 while another sweep is due
+    int    quirk;      // 1: accumulator never reset (Mode R, fluid); 2: solid variant
+};
+
+constexpr int RED_THREADS = 256;
+constexpr int MAX_RED_BLOCKS = 4096;
+
+// Multigrid level (Mode C pressure / energy operators on coarser grids).
+struct MGLevel;
+
+struct hs_ctx {
+    hs_params prm;
+    Dims d;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool inited = false;
+    bool have_matrix = false;
+
+    // host copies
+    std::vector<double> X, Y, Z;
+    std::vector<uint8_t> type;          // C layout
+    int solid_regions = 0;
+    int sb[6] = {0, -1, 0, -1, 0, -1};  // solid bbox (padded, inclusive)
+    double n_solved[3] = {0, 0, 0};     // Mode C solved-face counts
+    double n_tcells = 0;                // Mode C energy cells (fluid + solid)
+
+    // device geometry and classes
+    double* geo = nullptr;              // 9 arrays of length max(n)+3 packed
+    Geo g{};
+    uint8_t* cls = nullptr;             // P layout CELL_*
+
+    // fields
+    double *u = nullptr, *v = nullptr, *w = nullptr, *up = nullptr, *vp = nullptr, *wp = nullptr;
+    double *p = nullptr, *psnap = nullptr, *ptemp = nullptr, *pprime = nullptr, *tf = nullptr, *ts = nullptr;
+    double *q = nullptr, *k = nullptr, *b = nullptr;
+    // p' operator: Mode R per-cell row coefficients (P), Mode C per-face (U/V/W)
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    // Krylov work (P layout)
+    double *kr = nullptr, *kp = nullptr, *ks = nullptr, *kx = nullptr, *kz = nullptr, *kq = nullptr;
+    // per-point squared change for lexicographic sweeps (max layout size)
+    double* dres = nullptr;
+    // reductions
+    double* partials = nullptr;
+    unsigned int* counter = nullptr;
+    SolveCtl* sctl = nullptr;
+    SweepCtl* wctl = nullptr;
+    SolveCtl* h_sctl = nullptr;         // pinned mirrors
+    SweepCtl* h_wctl = nullptr;
+    // interface pairs (Mode R)
+    long* if_fluid = nullptr;
+    long* if_solid = nullptr;
+    long n_if = 0;
+
+    // multigrid hierarchy (created on demand)
+    std::vector<MGLevel*> mg;
+    int last_iters = 0;
+};
+
+// ---- kernel launchers (defined in the .hip files) ----
+int launch_fill(hs_ctx* c, double* a, long n, double v);
+int launch_copy(hs_ctx* c, double* dst, const double* src, long n);
+int bc_x_ref(hs_ctx* c);
+int bc_y_ref(hs_ctx* c);
+int bc_z_ref(hs_ctx* c);
+int cell_bc_ref(hs_ctx* c);
+int ptemp_from_p_all(hs_ctx* c);
+int bc_cfg_init(hs_ctx* c);
+int bc_cfg_outflow(hs_ctx* c, int comp);
+int zero_interface_faces(hs_ctx* c);
+int momentum_sweep(hs_ctx* c, int comp);            // one sweep incl. residual -> wctl
+int sweep_begin(hs_ctx* c, double n, double tol, int maxsteps, int quirk);
+int assemble_matrix(hs_ctx* c);
+int assemble_rhs(hs_ctx* c);
+int spmv(hs_ctx* c, const double* x, double* y);
+int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
+int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
+int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);
+int correct_velocity(hs_ctx* c);
+int update_pressure(hs_ctx* c);
+int energy_sweep(hs_ctx* c);
+int solid_sweep(hs_ctx* c);
+int exchange_kernel(hs_ctx* c);
+int energy_balance(hs_ctx* c, double* heat, double* resid);
+int gmg_setup(hs_ctx* c);
+int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
+void gmg_free(hs_ctx* c);

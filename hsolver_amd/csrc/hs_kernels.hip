@@ -1,0 +1,1179 @@
+// hs_kernels.hip — gfx950 kernels of the SIMPLE conjugate-heat-transfer step.
+//
+// Every point update is written in the same operation order as the reference loop
+// it replaces (built with -ffp-contract=off), so a sweep run in the reference's
+// lexicographic order reproduces the reference's values bit for bit:
+//   lexicographic Gauss-Seidel on a 7-point stencil == hyperplane (i+j+k = s)
+//   wavefront, because every point reads the updated i-1/j-1/k-1 neighbours
+//   (plane s-1) and the old i+1/j+1/k+1 neighbours (plane s+1).
+// Red-black ordering is the fast path: two fully parallel colour passes per sweep.
+//
+// Reductions are deterministic: per-block partial sums, then the last block to
+// finish (agent-scope acq_rel counter) adds the partials in a fixed order and runs
+// the scalar control step (CG alpha/beta/stop rule, Gauss-Seidel stop rule) on the
+// device, so no host round trip is needed per iteration.
+#include <cfloat>
+#include <cmath>
+
+#include "hs_internal.h"
+
+namespace {
+
+constexpr int BX = 64, BY = 4, BT = BX * BY;
+constexpr int FLAT = 256;
+constexpr int FLAT_BLOCKS = 2048;
+
+__device__ __forceinline__ double posd(double a) { return (a < 0.0) ? 0.0 : a; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block sum over a block of NT threads (NT multiple of 64); result valid in tid 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, int tid) {
+    __shared__ double sh[NT / 64];
+    v = wave_sum(v);
+    const int lane = tid & 63, wid = tid >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (wid == 0) {
+        r = (lane < NT / 64) ? sh[lane] : 0.0;
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Writes the block partial; returns true in every thread of the block that finished
+// last, with the full sum in *out (valid in tid 0).
+template <int NT>
+__device__ bool reduce_last(double v, int tid, int bid, int nblocks, double* partials,
+                            unsigned int* counter, double* out) {
+    __shared__ int am_last;
+    const double bs = block_sum<NT>(v, tid);
+    if (tid == 0) {
+        __hip_atomic_store(&partials[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double s = 0.0;
+    for (int i = tid; i < nblocks; i += NT)
+        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = block_sum<NT>(s, tid);
+    if (tid == 0) {
+        *out = s;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+// Gauss-Seidel loop control of solveX/Y/Zmomentum :580-658, solveTemp :797-880
+// (quirk 1: accumulator starts at 1, divided by N, never reset) and
+// MobSolver::solveTemperature mobs_solver.h:470-499 (quirk 2).
+__device__ void sweep_control(SweepCtl* w, double s) {
+    w->step += 1;
+    if (w->quirk == 1) {
+        w->residual = (w->residual + s) / w->ncount;
+        w->global = sqrt(w->residual);
+    } else if (w->quirk == 2) {
+        w->residual = sqrt((w->residual + s) / w->ncount);
+        w->global = w->residual;
+    } else {
+        w->residual = s;
+        w->global = sqrt(s / w->ncount);
+    }
+    w->active = (w->step < w->maxsteps) && !(w->global < w->tol) ? 1 : 0;
+}
+
+// ------------------------------------------------------------- fills ---
+__global__ void k_fill(double* a, long n, double v) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) a[t] = v;
+}
+__global__ void k_copy(double* dst, const double* src, long n) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
+}
+
+// --------------------------------------------------------------- BCs ---
+// applyXBoundaryCondition fluid_solver.h:209-257 (region spans the domain: only the
+// min-side ghost branch, Q10; east wall face zeroed).
+__global__ void k_bc_x_ref(Dims d, double* u, double* v, double* w) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (j < d.ny + 2 && k < d.nz + 2) {
+        u[idxU(d, 0, j, k)] = 0; v[idxV(d, 0, j, k)] = 0; w[idxW(d, 0, j, k)] = 0;
+        if (j < d.ny && k < d.nz) u[idxU(d, d.nx + 1, j + 1, k + 1)] = 0;
+    }
+}
+// applyYBoundaryCondition fluid_solver.h:284-322 (lid on the top interior u row, Q9).
+__global__ void k_bc_y_ref(Dims d, double* u, double* v, double* w, double ulid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i < d.nx + 2 && k < d.nz + 2) {
+        u[idxU(d, i, 0, k)] = 0; v[idxV(d, i, 0, k)] = 0; w[idxW(d, i, 0, k)] = 0;
+    }
+    __syncthreads();
+    if (i < d.nx && k < d.nz) {
+        v[idxV(d, i + 1, d.ny + 1, k + 1)] = 0;
+        u[idxU(d, i + 1, d.ny, k + 1)] = ulid;
+    }
+}
+// applyZBoundaryCondition fluid_solver.h:259-282.
+__global__ void k_bc_z_ref(Dims d, double* u, double* v, double* w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < d.nx + 2 && j < d.ny + 2) {
+        u[idxU(d, i, j, 0)] = 0; v[idxV(d, i, j, 0)] = 0; w[idxW(d, i, j, 0)] = 0;
+        if (i < d.nx && j < d.ny) w[idxW(d, i + 1, j + 1, d.nz + 1)] = 0;
+    }
+}
+
+// applyCellBoundaryCondition fluid_solver.h:324-393.  The reference runs one sequential
+// k,j,i loop in which boundary cell c copies its inward neighbour.  A neighbour later in
+// the loop (the +x/+y/+z copies of min-side cells) is read before its own update, a
+// neighbour earlier in the loop (the -x/-y/-z copies of max-side cells) after it.  For
+// n >= 3 the dependency chains are at most x-max <- y-max <- z-max deep, so the loop is
+// exactly: pass 0 every min-side cell and every z-max cell from a snapshot, pass 1 the
+// y-max cells, pass 2 the x-max cells (each reading the current array).
+__device__ __forceinline__ int cellbc_branch(const Dims& d, int gx, int gy, int gz) {
+    if (gx == 0) return 1;
+    if (gx == d.nx - 1) return 2;
+    if (gy == 0) return 3;
+    if (gy == d.ny - 1) return 4;
+    if (gz == 0) return 5;
+    if (gz == d.nz - 1) return 6;
+    return 0;
+}
+__global__ void k_cellbc_pass(Dims d, double* p, const double* snap, double* pt, int pass) {
+    // one thread per cell of the 6 boundary planes: launch over (nx*ny) x 6 planes style
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nxy = (long)d.nx * d.ny, nxz = (long)d.nx * d.nz, nyz = (long)d.ny * d.nz;
+    const long total = 2 * (nxy + nxz + nyz);
+    if (t >= total) return;
+    int gx, gy, gz;
+    long r = t;
+    if (r < 2 * nyz) { gx = (r < nyz) ? 0 : d.nx - 1; r %= nyz; gy = r % d.ny; gz = r / d.ny; }
+    else if ((r -= 2 * nyz) < 2 * nxz) { gy = (r < nxz) ? 0 : d.ny - 1; r %= nxz; gx = r % d.nx; gz = r / d.nx; }
+    else { r -= 2 * nxz; gz = (r < nxy) ? 0 : d.nz - 1; r %= nxy; gx = r % d.nx; gy = r / d.nx; }
+    const int br = cellbc_branch(d, gx, gy, gz);
+    // a cell appears on up to three planes: handle it on the plane of its branch only
+    const bool on_x = (t < 2 * nyz);
+    const bool on_y = !on_x && (t < 2 * nyz + 2 * nxz);
+    const bool on_z = !on_x && !on_y;
+    if ((br == 1 || br == 2) != on_x) return;
+    if ((br == 3 || br == 4) != on_y) return;
+    if ((br == 5 || br == 6) != on_z) return;
+    const int want = (br == 4) ? 1 : (br == 2 ? 2 : 0);
+    if (want != pass) return;
+    const long c = idxC(d, gx, gy, gz);
+    double val;
+    switch (br) {
+        case 1: val = snap[idxC(d, gx + 1, gy, gz)]; break;
+        case 2: val = p[idxC(d, gx - 1, gy, gz)]; break;
+        case 3: val = snap[idxC(d, gx, gy + 1, gz)]; break;
+        case 4: val = p[idxC(d, gx, gy - 1, gz)]; break;
+        case 5: val = snap[idxC(d, gx, gy, gz + 1)]; break;
+        default: val = snap[idxC(d, gx, gy, gz - 1)]; break;
+    }
+    p[c] = val;
+    const int i = gx, j = gy, k = gz, cx = d.nx, cz = d.nz;
+    switch (br) {
+        case 1: pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i, j + 1, k + 1)] = val; break;
+        case 2: pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i + 2, j + 1, k + 1)] = val; break;
+        case 3: pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i + 1, j, k + 1)] = val; break;
+        case 4:
+            pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i + 1, j + 2, k + 1)] = val;
+            if (i + 1 == cx) pt[idxP(d, i + 2, j + 2, k + 1)] = val;
+            if (k + 1 == cz) pt[idxP(d, i + 1, j + 2, k + 2)] = val;
+            if (i + 1 == 1) pt[idxP(d, i, j + 2, k + 1)] = val;
+            if (k + 1 == 1) pt[idxP(d, i + 1, j + 2, k)] = val;
+            if (i + 1 == cx && k + 1 == cz) pt[idxP(d, i + 2, j + 2, k + 2)] = val;
+            if (i + 1 == 1 && k + 1 == 1) pt[idxP(d, i, j + 2, k)] = val;
+            break;
+        case 5: pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i + 1, j + 1, k)] = val; break;
+        default: pt[idxP(d, i + 1, j + 1, k + 1)] = val; pt[idxP(d, i + 1, j + 1, k + 2)] = val; break;
+    }
+}
+__global__ void k_zero_one(double* p) { p[0] = 0.0; }
+
+// p_temp interior = p for every cell (setForNext fluid_solver.h:1655-1660).
+__global__ void k_ptemp_all(Dims d, const double* p, double* pt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i < d.nx) pt[idxP(d, i + 1, j + 1, k + 1)] = p[idxC(d, i, j, k)];
+}
+
+// Mode C: inlet u on x=min faces of fluid cells, lid ghost row for u on y=max.
+__global__ void k_bc_cfg_init(Dims d, double* u, const uint8_t* cls, double uin, int lid, double ulid) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (j >= 1 && j <= d.ny && k >= 1 && k <= d.nz)
+        u[idxU(d, 1, j, k)] = (cls[idxP(d, 1, j, k)] == CELL_FLUID) ? uin : 0.0;
+    if (lid && k < d.nz + 2)
+        for (int i = j; i < d.nx + 3; i += blockDim.x * gridDim.x) u[idxU(d, i, d.ny + 1, k)] = ulid;
+}
+// Mode C zero-gradient outflow on x = max (u faces, v/w ghosts, temperature ghost).
+__global__ void k_bc_cfg_outflow(Dims d, double* u, double* v, double* w, double* t, const uint8_t* cls, int comp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+    const int cx = d.nx;
+    if (comp == 0) {
+        if (j <= d.ny && k <= d.nz)
+            u[idxU(d, cx + 1, j, k)] = (cls[idxP(d, cx, j, k)] == CELL_FLUID) ? u[idxU(d, cx, j, k)] : 0.0;
+    } else if (comp == 1) {
+        if (j >= 2 && j <= d.ny && k <= d.nz) v[idxV(d, cx + 1, j, k)] = v[idxV(d, cx, j, k)];
+    } else if (comp == 2) {
+        if (j <= d.ny && k >= 2 && k <= d.nz) w[idxW(d, cx + 1, j, k)] = w[idxW(d, cx, j, k)];
+    } else {
+        if (j <= d.ny && k <= d.nz) t[idxP(d, cx + 1, j, k)] = t[idxP(d, cx, j, k)];
+    }
+}
+
+// interface faces start at zero velocity (fluid_solver.h:1005, :1116, :1225)
+__global__ void k_zero_ib(Dims d, double* u, double* v, double* w, const uint8_t* cls) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i < 1 || j < 1 || k < 1) return;
+    auto diff = [&](long a, long b) {
+        const uint8_t ca = cls[a], cb = cls[b];
+        return (ca == CELL_FLUID && cb == CELL_SOLID) || (ca == CELL_SOLID && cb == CELL_FLUID);
+    };
+    if (i >= 2 && i <= d.nx && j <= d.ny && k <= d.nz && diff(idxP(d, i - 1, j, k), idxP(d, i, j, k))) u[idxU(d, i, j, k)] = 0;
+    if (j >= 2 && j <= d.ny && i <= d.nx && k <= d.nz && diff(idxP(d, i, j - 1, k), idxP(d, i, j, k))) v[idxV(d, i, j, k)] = 0;
+    if (k >= 2 && k <= d.nz && i <= d.nx && j <= d.ny && diff(idxP(d, i, j, k - 1), idxP(d, i, j, k))) w[idxW(d, i, j, k)] = 0;
+}
+
+// ----------------------------------------------------------- momentum ---
+struct MomArgs {
+    Dims d;
+    Geo g;
+    double *u, *v, *w, *ap;
+    const double* pt;
+    const uint8_t* cls;
+    double* dres;
+    double c05, c2nu, relax;
+    int modeR;
+};
+
+__device__ __forceinline__ bool is_fluid(const MomArgs& a, long i, long j, long k) {
+    return a.cls[idxP(a.d, i, j, k)] == CELL_FLUID;
+}
+
+// One face of solveX/Y/Zmomentum (fluid_solver.h:596-625 / :513-542 / :427-457).
+// Returns 1 if the face value was updated, 2 if only its diagonal was written, 0 if
+// untouched; *dd receives the squared change.
+template <int COMP>
+__device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long k, double* dd) {
+    const Dims& d = a.d;
+    bool active, touch;
+    if (a.modeR) {
+        active = is_fluid(a, i, j, k);
+        touch = false;
+    } else {
+        const bool f1 = is_fluid(a, i, j, k);
+        const bool f0 = COMP == 0 ? is_fluid(a, i - 1, j, k) : (COMP == 1 ? is_fluid(a, i, j - 1, k) : is_fluid(a, i, j, k - 1));
+        active = f0 && f1;
+        touch = f0 || f1;
+    }
+    if (!active && !touch) return 0;
+    const double xp = a.g.xp[i], xw = a.g.xw[i], xe = a.g.xe[i];
+    const double yp = a.g.yp[j], ys = a.g.ys[j], yn = a.g.yn[j];
+    const double zp = a.g.zp[k], zf = a.g.zf[k], zb = a.g.zb[k];
+    const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
+    const double Dw = a.c2nu * Ayz / (xp + xw), De = a.c2nu * Ayz / (xp + xe);
+    const double Ds = a.c2nu * Axz / (ys + yp), Dn = a.c2nu * Axz / (yn + yp);
+    const double Db = a.c2nu * Axy / (zb + zp), Df = a.c2nu * Axy / (zf + zp);
+    const double* U = a.u;
+    const double* V = a.v;
+    const double* W = a.w;
+    double Fmx, Fpx, Fmy, Fpy, Fmz, Fpz, b;
+    long s0, sy, sz;
+    double* X;
+    const double c05 = a.c05;
+    if (COMP == 0) {
+        s0 = idxU(d, i, j, k); sy = d.sUy; sz = d.sUz; X = a.u;
+        Fmx = c05 * Ayz * (U[s0] + U[s0 - 1]);
+        Fpx = c05 * Ayz * (U[s0] + U[s0 + 1]);
+        Fmy = c05 * Axz * (V[idxV(d, i, j, k)] + V[idxV(d, i - 1, j, k)]);
+        Fpy = c05 * Axz * (V[idxV(d, i, j + 1, k)] + V[idxV(d, i - 1, j + 1, k)]);
+        Fpz = c05 * Axy * (W[idxW(d, i, j, k + 1)] + W[idxW(d, i - 1, j, k + 1)]);
+        Fmz = c05 * Axy * (W[idxW(d, i, j, k)] + W[idxW(d, i - 1, j, k)]);
+        b = Ayz * (a.pt[idxP(d, i - 1, j, k)] - a.pt[idxP(d, i, j, k)]);
+    } else if (COMP == 1) {
+        s0 = idxV(d, i, j, k); sy = d.sVy; sz = d.sVz; X = a.v;
+        Fmy = c05 * Axz * (V[s0] + V[s0 - sy]);
+        Fpy = c05 * Axz * (V[s0] + V[s0 + sy]);
+        Fmx = c05 * Ayz * (U[idxU(d, i, j, k)] + U[idxU(d, i, j - 1, k)]);
+        Fpx = c05 * Ayz * (U[idxU(d, i + 1, j, k)] + U[idxU(d, i + 1, j - 1, k)]);
+        Fpz = c05 * Axy * (W[idxW(d, i, j, k + 1)] + W[idxW(d, i, j - 1, k + 1)]);
+        Fmz = c05 * Axy * (W[idxW(d, i, j, k)] + W[idxW(d, i, j - 1, k)]);
+        b = Axz * (a.pt[idxP(d, i, j - 1, k)] - a.pt[idxP(d, i, j, k)]);
+    } else {
+        s0 = idxW(d, i, j, k); sy = d.sPy; sz = d.sPz; X = a.w;
+        Fmz = c05 * Axy * (W[s0] + W[s0 - sz]);
+        Fpz = c05 * Axy * (W[s0] + W[s0 + sz]);
+        Fmx = c05 * Ayz * (U[idxU(d, i, j, k)] + U[idxU(d, i, j, k - 1)]);
+        Fpx = c05 * Ayz * (U[idxU(d, i + 1, j, k)] + U[idxU(d, i + 1, j, k - 1)]);
+        Fmy = c05 * Axz * (V[idxV(d, i, j, k)] + V[idxV(d, i, j, k - 1)]);
+        Fpy = c05 * Axz * (V[idxV(d, i, j + 1, k)] + V[idxV(d, i, j + 1, k - 1)]);
+        b = Axy * (a.pt[idxP(d, i, j, k - 1)] - a.pt[idxP(d, i, j, k)]);
+    }
+    // reference upwind labels: y-momentum swaps x and y, z-momentum swaps z (Q7)
+    const bool swx = a.modeR && COMP == 1, swy = a.modeR && COMP == 1, swz = a.modeR && COMP == 2;
+    const double ae = De + (swx ? posd(-Fmx) : posd(-Fpx));
+    const double aw = Dw + (swx ? posd(Fpx) : posd(Fmx));
+    const double an = Dn + (swy ? posd(-Fmy) : posd(-Fpy));
+    const double as = Ds + (swy ? posd(Fpy) : posd(Fmy));
+    const double ab = Db + (swz ? posd(-Fmz) : posd(-Fpz));
+    const double af = Df + (swz ? posd(Fpz) : posd(Fmz));
+    const double ap = ae + aw + an + as + ab + af;
+    a.ap[s0] = ap;
+    if (!active) return 2;
+    const double nv = (ae * X[s0 + 1] + aw * X[s0 - 1] + an * X[s0 + sy] + as * X[s0 - sy] +
+                       af * X[s0 - sz] + ab * X[s0 + sz] + b) / ap;
+    const double old = X[s0];
+    const double val = a.relax * nv + (1 - a.relax) * old;
+    X[s0] = val;
+    *dd = (val - old) * (val - old);
+    return 1;
+}
+
+// Red-black colour pass over the face box [1..ni] x [1..nj] x [1..nk]; per-block
+// partial of the squared change, control step in the last block after colour 1.
+template <int COMP>
+__global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour,
+                                               double* partials, unsigned* counter, SweepCtl* wctl) {
+    if (!wctl->active) return;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i <= ni && j <= nj) {
+        const int k0 = 1 + blockIdx.z * kc, k1 = min(nk, k0 + kc - 1);
+        for (int k = k0; k <= k1; ++k) {
+            if (((i + j + k) & 1) != colour) continue;
+            double dd = 0.0;
+            if (mom_update<COMP>(a, i, j, k, &dd) == 1) acc += dd;
+        }
+    }
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    double* part = partials + (colour ? nb : 0);
+    double tot;
+    if (colour == 0) {
+        const double bs = block_sum<BT>(acc, tid);
+        if (tid == 0) part[bid] = bs;
+        return;
+    }
+    // colour 1: combine both colours' partials (fixed order) in the last block
+    const double bs = block_sum<BT>(acc, tid);
+    __shared__ int am_last;
+    if (tid == 0) {
+        __hip_atomic_store(&part[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int t = tid; t < 2 * nb; t += BT) s += __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot = block_sum<BT>(s, tid);
+    if (tid == 0) {
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sweep_control(wctl, tot);
+    }
+}
+
+// One hyperplane i+j+k = s of the lexicographic sweep; dres gets the squared change.
+template <int COMP>
+__global__ __launch_bounds__(256) void k_mom_plane(MomArgs a, int ni, int nj, int nk, int s, SweepCtl* wctl, long nlayout) {
+    if (!wctl->active) return;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int j = 1 + (int)(t % nj), k = 1 + (int)(t / nj);
+    if (k > nk) return;
+    const int i = s - j - k;
+    if (i < 1 || i > ni) return;
+    double dd = 0.0;
+    const int r = mom_update<COMP>(a, i, j, k, &dd);
+    if (r == 1) {
+        const long s0 = COMP == 0 ? idxU(a.d, i, j, k) : (COMP == 1 ? idxV(a.d, i, j, k) : idxW(a.d, i, j, k));
+        a.dres[s0] = dd;
+    }
+    (void)nlayout;
+}
+
+// Deterministic sum of dres[0..n) followed by the sweep control step.
+__global__ __launch_bounds__(FLAT) void k_sum_control(const double* x, long n, double* partials, unsigned* counter,
+                                                      SweepCtl* wctl) {
+    if (!wctl->active) return;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) acc += x[t];
+    double tot;
+    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0)
+        sweep_control(wctl, tot);
+}
+
+// ------------------------------------------------------------- energy ---
+struct TArgs {
+    Dims d;
+    Geo g;
+    const double *u, *v, *w;
+    double* t;
+    const double *q, *kc;
+    const uint8_t* cls;
+    double* dres;
+    double alpha, relax, rho;
+    int modeR;
+    int solid;   // 1: MobSolver::solveTemperature on solid cells (Mode R)
+};
+
+// solveTemp point (fluid_solver.h:807-846, Q11).
+__device__ __forceinline__ double t_point_ref(const TArgs& a, long i, long j, long k) {
+    const Dims& d = a.d;
+    const double* T = a.t;
+    const double xp = a.g.xp[i], xw = a.g.xw[i];
+    const double yp = a.g.yp[j], ys = a.g.ys[j];
+    const double zp = a.g.zp[k], zf = a.g.zf[k];
+    const double sx = (xp + xw) / 2.0, sy = (yp + ys) / 2.0, sz = (zp + zf) / 2.0;
+    const long c = idxP(d, i, j, k), sY = d.sPy, sZ = d.sPz;
+    const double al = a.alpha;
+    const double conv = -a.u[idxU(d, i, j, k)] * (T[c + 1] - T[c - 1])
+                        - a.v[idxV(d, i, j, k)] * (T[c + sY] - T[c - sY])
+                        - a.w[idxW(d, i, j, k)] * (T[c + sZ] - T[c - sZ]);
+    const double diff = al * (T[c + 1] + T[c - 1]) / sx / sx
+                        + al * (T[c + sY] + T[c - sY]) / sy / sy
+                        + al * (T[c + sZ] + T[c - sZ]) / sz / sz;
+    const double denom = 2 * al * (1 / sx / sx + 1 / sy / sy + 1 / sz / sz);
+    return (conv + diff) / denom;
+}
+
+// Mode C conjugate energy coefficients (harmonic face conductance, upwind convection).
+__device__ __forceinline__ double t_coeffs_cfg(const TArgs& a, long i, long j, long k, double cf[6], double* bs) {
+    const Dims& d = a.d;
+    const double xp = a.g.xp[i], yp = a.g.yp[j], zp = a.g.zp[k];
+    const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
+    const double kP = a.kc[idxC(d, i - 1, j - 1, k - 1)];
+    auto kof = [&](long ii, long jj, long kk) {
+        if (ii < 1 || ii > d.nx || jj < 1 || jj > d.ny || kk < 1 || kk > d.nz) return kP;
+        return a.kc[idxC(d, ii - 1, jj - 1, kk - 1)];
+    };
+    const double dW = (i - 1 < 1) ? xp : a.g.xp[i - 1], dE = (i + 1 > d.nx) ? xp : a.g.xp[i + 1];
+    const double dS = (j - 1 < 1) ? yp : a.g.yp[j - 1], dN = (j + 1 > d.ny) ? yp : a.g.yp[j + 1];
+    const double dB = (k - 1 < 1) ? zp : a.g.zp[k - 1], dT = (k + 1 > d.nz) ? zp : a.g.zp[k + 1];
+    const double DW = Ayz / (0.5 * xp / kP + 0.5 * dW / kof(i - 1, j, k));
+    const double DE = Ayz / (0.5 * xp / kP + 0.5 * dE / kof(i + 1, j, k));
+    const double DS = Axz / (0.5 * yp / kP + 0.5 * dS / kof(i, j - 1, k));
+    const double DN = Axz / (0.5 * yp / kP + 0.5 * dN / kof(i, j + 1, k));
+    const double DB = Axy / (0.5 * zp / kP + 0.5 * dB / kof(i, j, k - 1));
+    const double DT = Axy / (0.5 * zp / kP + 0.5 * dT / kof(i, j, k + 1));
+    const double rho = a.rho;
+    const double FW = rho * Ayz * a.u[idxU(d, i, j, k)], FE = rho * Ayz * a.u[idxU(d, i + 1, j, k)];
+    const double FS = rho * Axz * a.v[idxV(d, i, j, k)], FN = rho * Axz * a.v[idxV(d, i, j + 1, k)];
+    const double FB = rho * Axy * a.w[idxW(d, i, j, k)], FT = rho * Axy * a.w[idxW(d, i, j, k + 1)];
+    cf[0] = DE + posd(-FE);
+    cf[1] = DW + posd(FW);
+    cf[2] = DN + posd(-FN);
+    cf[3] = DS + posd(FS);
+    cf[4] = DB + posd(FB);
+    cf[5] = DT + posd(-FT);
+    *bs = a.q[idxC(d, i - 1, j - 1, k - 1)] * (xp * yp * zp);
+    return cf[0] + cf[1] + cf[2] + cf[3] + cf[4] + cf[5];
+}
+
+__device__ __forceinline__ bool t_update(const TArgs& a, long i, long j, long k, double* dd) {
+    const Dims& d = a.d;
+    const long c = idxP(d, i, j, k);
+    const uint8_t cl = a.cls[c];
+    double* T = a.t;
+    double nv;
+    if (a.solid) {
+        if (cl != CELL_SOLID) return false;
+        // MobSolver::solveTemperature mobs_solver.h:484-490 (Q12)
+        nv = 1.0 / 6.0 * (T[c + 1] + T[c - 1] + T[c - d.sPy] + T[c + d.sPy] + T[c - d.sPz] + T[c + d.sPz]);
+    } else if (a.modeR) {
+        if (cl != CELL_FLUID) return false;
+        nv = t_point_ref(a, i, j, k);
+    } else {
+        if (cl == CELL_GHOST) return false;
+        double cf[6], bs;
+        const double ap = t_coeffs_cfg(a, i, j, k, cf, &bs);
+        nv = (cf[0] * T[c + 1] + cf[1] * T[c - 1] + cf[2] * T[c + d.sPy] + cf[3] * T[c - d.sPy] +
+              cf[4] * T[c - d.sPz] + cf[5] * T[c + d.sPz] + bs) / ap;
+    }
+    const double old = T[c];
+    const double val = nv * a.relax + (1 - a.relax) * old;
+    T[c] = val;
+    *dd = (val - old) * (val - old);
+    return true;
+}
+
+__global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int kc, int colour,
+                                             double* partials, unsigned* counter, SweepCtl* wctl) {
+    if (!wctl->active) return;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const long i = i0 + blockIdx.x * BX + threadIdx.x, j = j0 + blockIdx.y * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i < i0 + ni && j < j0 + nj) {
+        const int ka = k0 + blockIdx.z * kc, kb = min(k0 + nk - 1, ka + kc - 1);
+        for (int k = ka; k <= kb; ++k) {
+            if (((i + j + k) & 1) != colour) continue;
+            double dd = 0.0;
+            if (t_update(a, i, j, k, &dd)) acc += dd;
+        }
+    }
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    double* part = partials + (colour ? nb : 0);
+    const double bs = block_sum<BT>(acc, tid);
+    if (colour == 0) {
+        if (tid == 0) part[bid] = bs;
+        return;
+    }
+    __shared__ int am_last;
+    if (tid == 0) {
+        __hip_atomic_store(&part[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int t = tid; t < 2 * nb; t += BT) s += __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double tot = block_sum<BT>(s, tid);
+    if (tid == 0) {
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sweep_control(wctl, tot);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_t_plane(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int s,
+                                                 SweepCtl* wctl) {
+    if (!wctl->active) return;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int jj = (int)(t % nj), kk = (int)(t / nj);
+    if (kk >= nk) return;
+    const int ii = s - jj - kk;
+    if (ii < 0 || ii >= ni) return;
+    const long i = i0 + ii, j = j0 + jj, k = k0 + kk;
+    double dd = 0.0;
+    if (t_update(a, i, j, k, &dd)) a.dres[idxP(a.d, i, j, k)] = dd;
+}
+
+// solver.h:62-148: the fluid's solid-side cells take the solid temperature and the
+// solid's fluid-side ghosts take the fluid temperature (both from pre-exchange values;
+// the written sets are disjoint from the read sets).
+__global__ void k_exchange(const long* fi, const long* si, long n, double* tf, double* ts) {
+    const long a = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const long F = fi[a], S = si[a];
+    const double from_solid = ts[S], from_fluid = tf[F];
+    tf[S] = from_solid;
+    ts[F] = from_fluid;
+}
+
+// ------------------------------------------------------ p' operator ---
+// Mode R row coefficients (initColsAndSetValues fluid_solver.h:1966-1996).
+__global__ void k_assemble_ref(Dims d, Geo g, const uint8_t* cls, const double* up, const double* vp, const double* wp,
+                               double* ax, double* ay, double* az) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > d.nx) return;
+    const long c = idxP(d, i, j, k);
+    if (cls[c] != CELL_FLUID) { ax[c] = 0; ay[c] = 0; az[c] = 0; return; }
+    const double xp = g.xp[i], yp = g.yp[j], zp = g.zp[k];
+    const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
+    ax[c] = Ayz / up[idxU(d, i, j, k)];
+    ay[c] = Axz / vp[idxV(d, i, j, k)];
+    az[c] = Axy / wp[idxW(d, i, j, k)];
+}
+// Mode C face coefficients rho A^2 / ap on every face touching a fluid cell.
+__global__ void k_assemble_cfg(Dims d, Geo g, const uint8_t* cls, const double* up, const double* vp, const double* wp,
+                               double* ax, double* ay, double* az, double rho) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    auto fl = [&](long ii, long jj, long kk) { return cls[idxP(d, ii, jj, kk)] == CELL_FLUID; };
+    if (i <= d.nx + 1 && j <= d.ny && k <= d.nz) {
+        const long f = idxU(d, i, j, k);
+        double v = 0.0;
+        if (fl(i - 1, j, k) || fl(i, j, k)) { const double A = g.yp[j] * g.zp[k]; v = rho * A * A / up[f]; }
+        ax[f] = v;
+    }
+    if (i <= d.nx && j <= d.ny + 1 && k <= d.nz) {
+        const long f = idxV(d, i, j, k);
+        double v = 0.0;
+        if (fl(i, j - 1, k) || fl(i, j, k)) { const double A = g.xp[i] * g.zp[k]; v = rho * A * A / vp[f]; }
+        ay[f] = v;
+    }
+    if (i <= d.nx && j <= d.ny && k <= d.nz + 1) {
+        const long f = idxW(d, i, j, k);
+        double v = 0.0;
+        if (fl(i, j, k - 1) || fl(i, j, k)) { const double A = g.xp[i] * g.yp[j]; v = rho * A * A / wp[f]; }
+        az[f] = v;
+    }
+}
+// right-hand side (fluid_solver.h:1998-2002 Mode R; area-weighted in Mode C)
+__global__ void k_rhs(Dims d, Geo g, const uint8_t* cls, const double* u, const double* v, const double* w,
+                      double* b, int modeR, double rho) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > d.nx) return;
+    const long c = idxP(d, i, j, k);
+    if (cls[c] != CELL_FLUID) { b[c] = 0.0; return; }
+    const double du = u[idxU(d, i + 1, j, k)] - u[idxU(d, i, j, k)];
+    const double dv = v[idxV(d, i, j + 1, k)] - v[idxV(d, i, j, k)];
+    const double dw = w[idxW(d, i, j, k + 1)] - w[idxW(d, i, j, k)];
+    if (modeR) b[c] = -du - dv - dw;
+    else {
+        const double xp = g.xp[i], yp = g.yp[j], zp = g.zp[k];
+        b[c] = -rho * (yp * zp * du + xp * zp * dv + xp * yp * dw);
+    }
+}
+
+struct OpArgs {
+    Dims d;
+    const double *ax, *ay, *az;
+    const uint8_t* cls;
+    int modeR;
+};
+
+// y = A x at one fluid cell: diagonal first, then B, S, W, E, N, T (CSR column order
+// of fluid_solver.h:2003-2083).  Entries of x outside fluid cells are zero.
+__device__ __forceinline__ double apply_row(const OpArgs& o, const double* x, long i, long j, long k, long c) {
+    const Dims& d = o.d;
+    double cB, cS, cW, cE, cN, cT, dg;
+    if (o.modeR) {
+        cW = cE = o.ax[c]; cS = cN = o.ay[c]; cB = cT = o.az[c];
+        dg = cE + cW + cN + cS + cT + cB;
+    } else {
+        cW = o.ax[idxU(d, i, j, k)]; cE = o.ax[idxU(d, i + 1, j, k)];
+        cS = o.ay[idxV(d, i, j, k)]; cN = o.ay[idxV(d, i, j + 1, k)];
+        cB = o.az[idxW(d, i, j, k)]; cT = o.az[idxW(d, i, j, k + 1)];
+        dg = cW + cE + cS + cN + cB + cT;
+    }
+    double t = dg * x[c];
+    t += (-cB) * x[c - d.sPz];
+    t += (-cS) * x[c - d.sPy];
+    t += (-cW) * x[c - 1];
+    t += (-cE) * x[c + 1];
+    t += (-cN) * x[c + d.sPy];
+    t += (-cT) * x[c + d.sPz];
+    return t;
+}
+
+// s = A p (z-marching columns) with the partial of <s,p>; last block: alpha.
+__global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, double* s, int kc, double* partials,
+                                                unsigned* counter, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const Dims& d = o.d;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
+        for (int k = k0; k <= k1; ++k) {
+            const long c = idxP(d, i, j, k);
+            if (o.cls[c] != CELL_FLUID) continue;
+            const double y = apply_row(o, p, i, j, k, c);
+            s[c] = y;
+            acc += y * p[c];
+        }
+    }
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    double tot;
+    if (reduce_last<BT>(acc, tid, bid, nb, partials, counter, &tot) && tid == 0) {
+        ctl->sdotp = tot;
+        if (tot == 0.0) { ctl->done = 1; return; }
+        const double alpha = ctl->gamma / tot;
+        if (!(alpha > DBL_MIN)) { ctl->done = 1; return; }
+        ctl->alpha = alpha;
+    }
+}
+
+// x += alpha p, r -= alpha s, partial <r,r>; last block: HYPRE stop rule and beta.
+__global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* x, double* r, const double* p, const double* s,
+                                                    double* partials, unsigned* counter, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const double alpha = ctl->alpha;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        x[t] += alpha * p[t];
+        const double rv = r[t] - alpha * s[t];
+        r[t] = rv;
+        acc += rv * rv;
+    }
+    double tot;
+    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
+        ctl->iters += 1;
+        ctl->gamma_old = ctl->gamma;
+        ctl->gamma = tot;
+        ctl->rr = tot;
+        if (tot / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; return; }
+        if (ctl->iters >= ctl->maxit) { ctl->done = 1; return; }
+        ctl->beta = ctl->gamma / ctl->gamma_old;
+    }
+}
+
+// p = r + beta p
+__global__ __launch_bounds__(FLAT) void k_cg_pupdate(long n, const double* r, double* p, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) p[t] = r[t] + beta * p[t];
+}
+
+// generic dot into ctl->red[slot]
+__global__ __launch_bounds__(FLAT) void k_dot(long n, const double* a, const double* b, double* partials, unsigned* counter,
+                                              double* out) {
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) acc += a[t] * b[t];
+    double tot;
+    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) *out = tot;
+}
+
+// CG setup: r = p = b, x = 0 (x0 = 0, fluid_solver.h:1766); <b,b> -> ctl
+__global__ __launch_bounds__(FLAT) void k_cg_setup(long n, const double* b, double* x, double* r, double* p,
+                                                   double* partials, unsigned* counter, SolveCtl* ctl) {
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        const double bv = b[t];
+        x[t] = 0.0; r[t] = bv; p[t] = bv;
+        acc += bv * bv;
+    }
+    double tot;
+    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
+        ctl->bb = tot;
+        ctl->gamma = tot;
+        ctl->rr = tot;
+        ctl->iters = 0;
+        ctl->converged = 0;
+        ctl->done = (tot == 0.0) ? 1 : 0;
+        if (tot == 0.0) ctl->converged = 1;
+    }
+}
+
+// plain y = A x (fluid cells) without reduction
+__global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* y, int kc) {
+    const Dims& d = o.d;
+    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > d.nx || j > d.ny) return;
+    const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
+    for (int k = k0; k <= k1; ++k) {
+        const long c = idxP(d, i, j, k);
+        y[c] = (o.cls[c] == CELL_FLUID) ? apply_row(o, x, i, j, k, c) : 0.0;
+    }
+}
+
+// zero entries outside fluid cells
+__global__ void k_mask_fluid(long n, const uint8_t* cls, double* x) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        if (cls[t] != CELL_FLUID) x[t] = 0.0;
+}
+// r = b - A x entries for the residual check
+__global__ void k_axpby(long n, double a, const double* x, double b, const double* y, double* z) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) z[t] = a * x[t] + b * y[t];
+}
+
+// ------------------------------------------------------- corrections ---
+// velocity correction fluid_solver.h:1459-1571 (Mode R: faces whose plus-side cell is
+// fluid; Mode C: faces between two fluid cells)
+__global__ void k_correct(Dims d, Geo g, const uint8_t* cls, const double* pp, double* u, double* v, double* w,
+                          const double* up, const double* vp, const double* wp, int modeR) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    auto fl = [&](long ii, long jj, long kk) { return cls[idxP(d, ii, jj, kk)] == CELL_FLUID; };
+    const int ex = modeR ? 0 : 1;
+    if (i <= d.nx + ex && j <= d.ny && k <= d.nz) {
+        const bool act = modeR ? fl(i, j, k) : (fl(i - 1, j, k) && fl(i, j, k));
+        if (act) {
+            const double faceyz = g.yp[j] * g.zp[k];
+            double ap = up[idxU(d, i, j, k)];
+            if (ap == 0) ap = 1e-10;
+            const double corr = (pp[idxP(d, i - 1, j, k)] - pp[idxP(d, i, j, k)]) * faceyz / ap;
+            u[idxU(d, i, j, k)] = u[idxU(d, i, j, k)] + corr;
+        }
+    }
+    if (i <= d.nx && j <= d.ny + ex && k <= d.nz) {
+        const bool act = modeR ? fl(i, j, k) : (fl(i, j - 1, k) && fl(i, j, k));
+        if (act) {
+            const double facexz = g.xp[i] * g.zp[k];
+            double ap = vp[idxV(d, i, j, k)];
+            if (ap == 0) ap = 1e-10;
+            const double corr = (pp[idxP(d, i, j - 1, k)] - pp[idxP(d, i, j, k)]) * facexz / ap;
+            v[idxV(d, i, j, k)] = v[idxV(d, i, j, k)] + corr;
+        }
+    }
+    if (i <= d.nx && j <= d.ny && k <= d.nz + ex) {
+        const bool act = modeR ? fl(i, j, k) : (fl(i, j, k - 1) && fl(i, j, k));
+        if (act) {
+            const double facexy = g.xp[i] * g.yp[j];
+            double ap = wp[idxW(d, i, j, k)];
+            if (ap == 0) ap = 1e-10;
+            const double corr = (pp[idxP(d, i, j, k - 1)] - pp[idxP(d, i, j, k)]) * facexy / ap;
+            w[idxW(d, i, j, k)] = w[idxW(d, i, j, k)] + corr;
+        }
+    }
+}
+// pressure update fluid_solver.h:1574-1598: p += relax p', p_temp = p on fluid cells.
+__global__ void k_update_p(Dims d, const uint8_t* cls, const double* pp, double* p, double* pt, double relax) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > d.nx) return;
+    const long c = idxP(d, i, j, k);
+    if (cls[c] != CELL_FLUID) return;
+    const long cc = idxC(d, i - 1, j - 1, k - 1);
+    const double nv = p[cc] + relax * pp[c];
+    p[cc] = nv;
+    pt[c] = nv;
+}
+
+// Mode C energy balance: sums of cell residuals and of heat input.
+__global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* partials, unsigned* counter, double* out) {
+    const Dims& d = a.d;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
+        for (int k = k0; k <= k1; ++k) {
+            const long c = idxP(d, i, j, k);
+            if (a.cls[c] == CELL_GHOST) continue;
+            double cf[6], bs;
+            const double ap = t_coeffs_cfg(a, i, j, k, cf, &bs);
+            const double* T = a.t;
+            acc += ap * T[c] - (cf[0] * T[c + 1] + cf[1] * T[c - 1] + cf[2] * T[c + d.sPy] + cf[3] * T[c - d.sPy] +
+                                cf[4] * T[c - d.sPz] + cf[5] * T[c + d.sPz]) - bs;
+        }
+    }
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    double tot;
+    if (reduce_last<BT>(acc, tid, bid, nb, partials, counter, &tot) && tid == 0) *out = tot;
+}
+
+__global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, double* partials, unsigned* counter, double* out) {
+    double acc = 0.0;
+    const long n = d.nC;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        const long i = t % d.nx, j = (t / d.nx) % d.ny, k = t / ((long)d.nx * d.ny);
+        acc += q[t] * (g.xp[i + 1] * g.yp[j + 1] * g.zp[k + 1]);
+    }
+    double tot;
+    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) *out = tot;
+}
+
+}  // namespace
+
+// =================================================================== launchers ===
+namespace {
+inline int kchunk(int nk, long tiles) {
+    // aim for ~4096 blocks
+    long want = 4096 / (tiles > 0 ? tiles : 1);
+    if (want < 1) want = 1;
+    int kc = (int)((nk + want - 1) / want);
+    return kc < 1 ? 1 : kc;
+}
+inline dim3 grid3(int ni, int nj, int nk, int kc) {
+    return dim3((ni + BX - 1) / BX, (nj + BY - 1) / BY, (nk + kc - 1) / kc);
+}
+MomArgs mom_args(hs_ctx* c, double* ap) {
+    MomArgs a;
+    a.d = c->d; a.g = c->g; a.u = c->u; a.v = c->v; a.w = c->w; a.ap = ap;
+    a.pt = c->ptemp; a.cls = c->cls; a.dres = c->dres;
+    a.c05 = 0.5 * c->prm.rho; a.c2nu = 2 * c->prm.nu; a.relax = c->prm.relax_u;
+    a.modeR = c->prm.mode == HS_MODE_REF;
+    return a;
+}
+TArgs t_args(hs_ctx* c, bool solid) {
+    TArgs a;
+    a.d = c->d; a.g = c->g; a.u = c->u; a.v = c->v; a.w = c->w;
+    a.t = solid ? c->ts : c->tf; a.q = c->q; a.kc = c->k; a.cls = c->cls; a.dres = c->dres;
+    a.alpha = c->prm.alpha_f; a.relax = solid ? c->prm.relax_ts : c->prm.relax_t; a.rho = c->prm.rho;
+    a.modeR = c->prm.mode == HS_MODE_REF; a.solid = solid ? 1 : 0;
+    return a;
+}
+OpArgs op_args(hs_ctx* c) {
+    OpArgs o;
+    o.d = c->d; o.ax = c->ax; o.ay = c->ay; o.az = c->az; o.cls = c->cls;
+    o.modeR = c->prm.mode == HS_MODE_REF;
+    return o;
+}
+}  // namespace
+
+int launch_fill(hs_ctx* c, double* a, long n, double v) {
+    k_fill<<<1024, 256, 0, c->stream>>>(a, n, v);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int launch_copy(hs_ctx* c, double* dst, const double* src, long n) {
+    k_copy<<<1024, 256, 0, c->stream>>>(dst, src, n);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int bc_x_ref(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_bc_x_ref<<<dim3((d.ny + 2 + 63) / 64, d.nz + 2), 64, 0, c->stream>>>(d, c->u, c->v, c->w);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int bc_y_ref(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_bc_y_ref<<<dim3((d.nx + 2 + 63) / 64, d.nz + 2), 64, 0, c->stream>>>(d, c->u, c->v, c->w, c->prm.u_lid);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int bc_z_ref(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_bc_z_ref<<<dim3((d.nx + 2 + 63) / 64, d.ny + 2), 64, 0, c->stream>>>(d, c->u, c->v, c->w);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int cell_bc_ref(hs_ctx* c) {
+    const Dims& d = c->d;
+    HS_TRY(launch_copy(c, c->psnap, c->p, d.nC));
+    const long total = 2 * ((long)d.nx * d.ny + (long)d.nx * d.nz + (long)d.ny * d.nz);
+    const int nb = (int)((total + 255) / 256);
+    for (int pass = 0; pass < 3; ++pass) {
+        k_cellbc_pass<<<nb, 256, 0, c->stream>>>(d, c->p, c->psnap, c->ptemp, pass);
+        HS_CHECK(hipGetLastError());
+    }
+    k_zero_one<<<1, 1, 0, c->stream>>>(c->p);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int ptemp_from_p_all(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_ptemp_all<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->p, c->ptemp);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int bc_cfg_init(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_bc_cfg_init<<<dim3((d.ny + 2 + 63) / 64, d.nz + 2), 64, 0, c->stream>>>(d, c->u, c->cls, c->prm.u_in, c->prm.lid,
+                                                                             c->prm.u_lid);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int bc_cfg_outflow(hs_ctx* c, int comp) {
+    if (!c->prm.outflow) return HS_OK;
+    const Dims& d = c->d;
+    k_bc_cfg_outflow<<<dim3((d.ny + 63) / 64, d.nz), 64, 0, c->stream>>>(d, c->u, c->v, c->w, c->tf, c->cls, comp);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int zero_interface_faces(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_zero_ib<<<dim3((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1), 64, 0, c->stream>>>(d, c->u, c->v, c->w, c->cls);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+int sweep_begin(hs_ctx* c, double n, double tol, int maxsteps, int quirk) {
+    SweepCtl* h = c->h_wctl;
+    h->residual = 1.0;
+    h->global = (quirk == 0) ? INFINITY : 1.0;
+    h->ncount = n;
+    h->tol = tol;
+    h->step = 0;
+    h->maxsteps = maxsteps;
+    h->quirk = quirk;
+    h->active = (maxsteps > 0 && !(h->global < tol)) ? 1 : 0;
+    HS_CHECK(hipMemcpyAsync(c->wctl, h, sizeof(SweepCtl), hipMemcpyHostToDevice, c->stream));
+    return HS_OK;
+}
+
+int momentum_sweep(hs_ctx* c, int comp) {
+    const Dims& d = c->d;
+    double* ap = comp == 0 ? c->up : (comp == 1 ? c->vp : c->wp);
+    MomArgs a = mom_args(c, ap);
+    const bool R = c->prm.mode == HS_MODE_REF;
+    int ni = d.nx, nj = d.ny, nk = d.nz;
+    if (!R) { if (comp == 0) ni += 1; else if (comp == 1) nj += 1; else nk += 1; }
+    if (c->prm.order == HS_ORDER_RB) {
+        const long tiles = (long)((ni + BX - 1) / BX) * ((nj + BY - 1) / BY);
+        const int kc = kchunk(nk, tiles);
+        const dim3 gr = grid3(ni, nj, nk, kc);
+        for (int colour = 0; colour < 2; ++colour) {
+            if (comp == 0) k_mom_rb<0><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            else if (comp == 1) k_mom_rb<1><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            else k_mom_rb<2><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            HS_CHECK(hipGetLastError());
+        }
+    } else {
+        const long nl = comp == 0 ? d.nU : (comp == 1 ? d.nV : d.nW);
+        HS_TRY(launch_fill(c, c->dres, nl, 0.0));
+        const long njk = (long)nj * nk;
+        const int nb = (int)((njk + 255) / 256);
+        for (int s = 3; s <= ni + nj + nk; ++s) {
+            if (comp == 0) k_mom_plane<0><<<nb, 256, 0, c->stream>>>(a, ni, nj, nk, s, c->wctl, nl);
+            else if (comp == 1) k_mom_plane<1><<<nb, 256, 0, c->stream>>>(a, ni, nj, nk, s, c->wctl, nl);
+            else k_mom_plane<2><<<nb, 256, 0, c->stream>>>(a, ni, nj, nk, s, c->wctl, nl);
+        }
+        HS_CHECK(hipGetLastError());
+        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, nl, c->partials, c->counter, c->wctl);
+        HS_CHECK(hipGetLastError());
+    }
+    return HS_OK;
+}
+
+static int t_sweep_box(hs_ctx* c, bool solid, int i0, int j0, int k0, int ni, int nj, int nk) {
+    const Dims& d = c->d;
+    TArgs a = t_args(c, solid);
+    if (c->prm.order == HS_ORDER_RB) {
+        const long tiles = (long)((ni + BX - 1) / BX) * ((nj + BY - 1) / BY);
+        const int kc = kchunk(nk, tiles);
+        const dim3 gr = grid3(ni, nj, nk, kc);
+        for (int colour = 0; colour < 2; ++colour) {
+            k_t_rb<<<gr, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            HS_CHECK(hipGetLastError());
+        }
+    } else {
+        HS_TRY(launch_fill(c, c->dres, d.nP, 0.0));
+        const long njk = (long)nj * nk;
+        const int nb = (int)((njk + 255) / 256);
+        for (int s = 0; s <= (ni - 1) + (nj - 1) + (nk - 1); ++s)
+            k_t_plane<<<nb, 256, 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, s, c->wctl);
+        HS_CHECK(hipGetLastError());
+        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, d.nP, c->partials, c->counter, c->wctl);
+        HS_CHECK(hipGetLastError());
+    }
+    return HS_OK;
+}
+
+int energy_sweep(hs_ctx* c) { return t_sweep_box(c, false, 1, 1, 1, c->d.nx, c->d.ny, c->d.nz); }
+
+int solid_sweep(hs_ctx* c) {
+    return t_sweep_box(c, true, c->sb[0], c->sb[2], c->sb[4], c->sb[1] - c->sb[0] + 1, c->sb[3] - c->sb[2] + 1,
+                       c->sb[5] - c->sb[4] + 1);
+}
+
+int exchange_kernel(hs_ctx* c) {
+    if (c->n_if == 0) return HS_OK;
+    k_exchange<<<(int)((c->n_if + 255) / 256), 256, 0, c->stream>>>(c->if_fluid, c->if_solid, c->n_if, c->tf, c->ts);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+int assemble_matrix(hs_ctx* c) {
+    const Dims& d = c->d;
+    if (c->prm.mode == HS_MODE_REF) {
+        k_assemble_ref<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->g, c->cls, c->up, c->vp, c->wp, c->ax,
+                                                                                  c->ay, c->az);
+    } else {
+        k_assemble_cfg<<<dim3((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1), 64, 0, c->stream>>>(
+            d, c->g, c->cls, c->up, c->vp, c->wp, c->ax, c->ay, c->az, c->prm.rho);
+    }
+    HS_CHECK(hipGetLastError());
+    c->have_matrix = true;
+    return HS_OK;
+}
+
+int assemble_rhs(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_rhs<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->g, c->cls, c->u, c->v, c->w, c->b,
+                                                                     c->prm.mode == HS_MODE_REF, c->prm.rho);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+int spmv(hs_ctx* c, const double* x, double* y) {
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    k_spmv<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(op_args(c), x, y, kc);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+int dot(hs_ctx* c, const double* a, const double* b, double* out_host) {
+    k_dot<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->d.nP, a, b, c->partials, c->counter, &c->sctl->red[0]);
+    HS_CHECK(hipGetLastError());
+    HS_CHECK(hipMemcpyAsync(out_host, &c->sctl->red[0], sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+// HYPRE ParCSR PCG with the identity preconditioner (fluid_solver.h:1789-1790, settings
+// :1871-1875), x0 = 0.  Iterations are launched in batches; the stop rule is evaluated
+// on the device and later launches of a finished solve return at entry.
+int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+    const Dims& d = c->d;
+    const long n = d.nP;
+    SolveCtl* h = c->h_sctl;
+    h->eps = c->prm.p_tol * c->prm.p_tol;
+    h->maxit = c->prm.p_maxit;
+    HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    k_cg_setup<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, b, x, c->kr, c->kp, c->partials, c->counter, c->sctl);
+    HS_CHECK(hipGetLastError());
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    const OpArgs o = op_args(c);
+    int launched = 0;
+    int batch = c->last_iters > 8 ? c->last_iters - 4 : 8;
+    while (true) {
+        for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
+            k_cg_spmv<<<gr, dim3(BX, BY), 0, c->stream>>>(o, c->kp, c->ks, kc, c->partials, c->counter, c->sctl);
+            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, c->kp, c->ks, c->partials, c->counter, c->sctl);
+            k_cg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, c->kr, c->kp, c->sctl);
+        }
+        HS_CHECK(hipGetLastError());
+        HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
+        HS_CHECK(hipStreamSynchronize(c->stream));
+        if (h->done || launched >= c->prm.p_maxit) break;
+        batch = 8;
+    }
+    c->last_iters = h->iters;
+    if (st) {
+        st->p_iters = h->iters;
+        st->p_converged = h->converged;
+        st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
+    }
+    if (!std::isfinite(h->rr)) return HS_E_NAN;
+    return HS_OK;
+}
+
+int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm) {
+    HS_TRY(spmv(c, x, c->kq));
+    k_axpby<<<1024, 256, 0, c->stream>>>(c->d.nP, 1.0, b, -1.0, c->kq, c->kq);
+    HS_CHECK(hipGetLastError());
+    double rr, bb;
+    HS_TRY(dot(c, c->kq, c->kq, &rr));
+    HS_TRY(dot(c, b, b, &bb));
+    *relres = bb > 0 ? sqrt(rr / bb) : 0.0;
+    *bnorm = sqrt(bb);
+    return HS_OK;
+}
+
+int correct_velocity(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_correct<<<dim3((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1), 64, 0, c->stream>>>(
+        d, c->g, c->cls, c->pprime, c->u, c->v, c->w, c->up, c->vp, c->wp, c->prm.mode == HS_MODE_REF);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+int update_pressure(hs_ctx* c) {
+    const Dims& d = c->d;
+    k_update_p<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->pprime, c->p, c->ptemp,
+                                                                         c->prm.relax_p);
+    HS_CHECK(hipGetLastError());
+    return launch_fill(c, c->pprime, d.nP, 0.0);
+}
+
+int energy_balance(hs_ctx* c, double* heat, double* resid) {
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    k_energy_balance<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials,
+                                                                                 c->counter, &c->sctl->red[1]);
+    HS_CHECK(hipGetLastError());
+    k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials, c->counter, &c->sctl->red[2]);
+    HS_CHECK(hipGetLastError());
+    double h2[2];
+    HS_CHECK(hipMemcpyAsync(h2, &c->sctl->red[1], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    *resid = h2[0];
+    *heat = h2[1];
+    return HS_OK;
+}
+
+int mask_fluid(hs_ctx* c, double* x) {
+    k_mask_fluid<<<1024, 256, 0, c->stream>>>(c->d.nP, c->cls, x);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}

@@ -1,0 +1,206 @@
+/*
+ * hsolver.h — C ABI of the MI355X (gfx950) SIMPLE conjugate-heat-transfer step.
+ *
+ * One hs_ctx owns the device copy of one structured, staggered, non-uniform grid:
+ * the fluid region (velocities on faces, pressure / pressure correction / fluid
+ * temperature on cells) and the solid ("mobs") temperature, on one GPU and one
+ * HIP stream.  Every entry point is stream-ordered and synchronises before it
+ * returns; host buffers are owned by the caller.  One ctx per host thread.
+ *
+ * The entry points mirror the member functions of the reference's header-only
+ * solver structs, which are the reference's only interface for this path:
+ *   FluidSolver  fluid_solver.h:52-182   (init :882, solve :1401, setForNext :1634)
+ *   MobSolver    mobs_solver.h:31-77     (init :116, solve :303, setForNext :290)
+ *   Solver       solver.h:24-45          (solve :44, exchange :62-148)
+ * Each declaration below names the reference function it replaces.
+ *
+ * Layouts (all x-fastest, "padded" = one ghost layer on every side, exactly the
+ * reference's padded work arrays, fluid_solver.h:93-109):
+ *   P  cells  (nx+2)(ny+2)(nz+2)   p_temp, p_prime, t_temp, solid t_temp
+ *   U  x-faces (nx+3)(ny+2)(nz+2)  u_star, up
+ *   V  y-faces (nx+2)(ny+3)(nz+2)  v_star, vp
+ *   W  z-faces (nx+2)(ny+2)(nz+3)  w_star, wp
+ *   C  cells  nx*ny*nz (unpadded)   LocalCell.data of p (distributed_mesh.h:17-20)
+ * Padded index of (i,j,k) in layout X with extents (ex,ey,ez) is i + ex*(j + ey*k).
+ */
+#ifndef HSOLVER_H
+#define HSOLVER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (the reference returns void everywhere; these replace its
+ *      silent failure modes, e.g. ignored PCG non-convergence :1792-1793) ---- */
+#define HS_OK          0
+#define HS_E_ARG      -1   /* bad argument / unsupported grid */
+#define HS_E_HIP      -2   /* HIP runtime error */
+#define HS_E_NOCONV   -3   /* linear solve hit maxit (only if params.fail_on_noconv) */
+#define HS_E_NAN      -4   /* NaN/Inf detected in a residual */
+#define HS_E_IO       -5   /* file could not be read */
+#define HS_E_STATE    -6   /* call out of order (e.g. solve before init) */
+#define HS_E_NOMEM    -7   /* device allocation failed */
+
+/* ---- physics mode ----
+ * HS_MODE_REF    reference-faithful: lid-driven cavity, every quirk of the
+ *                reference kept (frozen p' matrix, never-reset GS residual,
+ *                swapped upwind labels in y/z momentum, leaky min-side walls,
+ *                uniform-mean solid conduction, split fluid/solid temperature
+ *                with a one-cell ghost exchange).
+ * HS_MODE_CONFIG parameterised channel / cavity physics: inlet, outflow,
+ *                no-penetration at every wall and solid face, symmetric
+ *                area-consistent p' operator rebuilt every iteration,
+ *                monolithic conjugate energy equation with harmonic-mean
+ *                face conductivity and a per-cell heat source. */
+#define HS_MODE_REF     0
+#define HS_MODE_CONFIG  1
+
+/* sweep ordering of the under-relaxed momentum / energy sweeps */
+#define HS_ORDER_LEX    0   /* lexicographic Gauss-Seidel, identical to the reference's k,j,i loops */
+#define HS_ORDER_RB     1   /* red-black Gauss-Seidel (colour = (i+j+k)&1 of the padded index) */
+
+/* pressure-correction solver */
+#define HS_PRECOND_NONE 0   /* CG with identity preconditioner: HYPRE ParCSR PCG as coded */
+#define HS_PRECOND_GMG  1   /* flexible CG preconditioned by geometric multigrid */
+
+#define HS_PREC_FP64    0
+#define HS_PREC_MIXED   1   /* FP32 multigrid inside FP64 flexible CG */
+
+typedef struct {
+    int32_t nx, ny, nz;               /* cells */
+    const double *xv, *yv, *zv;       /* vertex coordinates, nx+1 / ny+1 / nz+1 (.in xgrid/ygrid/zgrid) */
+    const uint8_t *cell_type;         /* nx*ny*nz, x-fastest: 0 fluid, 1 solid */
+} hs_grid;
+
+typedef struct {
+    int32_t mode;                     /* HS_MODE_* */
+    int32_t order;                    /* HS_ORDER_* */
+    double  rho;                      /* RHO  (fluid_solver.h:23) */
+    double  nu;                       /* NU   (fluid_solver.h:25) */
+    double  alpha_f;                  /* ALPHA fluid diffusivity (fluid_solver.h:30); Mode C: k_fluid / (rho c_p) */
+    double  k_ratio;                  /* Mode C: k_solid / k_fluid (per-cell k overrides, hs_set_conductivity) */
+    double  u_lid;                    /* U0 lid speed (fluid_solver.h:22) */
+    double  u_in;                     /* Mode C: inlet u on the x=0 faces (0 = wall) */
+    int32_t outflow;                  /* Mode C: 1 = zero-gradient outflow on the x=max faces */
+    int32_t lid;                      /* Mode C: 1 = moving lid on the y=max wall */
+    double  relax_u, relax_p, relax_t, relax_ts;   /* UNDER_RELAX (fluid_solver.h:35, mobs_solver.h:27) */
+    double  mom_tol, t_tol, ts_tol;   /* GS stop: CONVERGENCE 2e-3 (fluid), CONVERGENCE2 1e-3 (solid) */
+    int32_t mom_sweeps_max, t_sweeps_max, ts_sweeps_max;   /* GS_STEP 100 */
+    double  p_tol;                    /* HYPRE_PCGSetTol (fluid_solver.h:1874) */
+    int32_t p_maxit;                  /* HYPRE_PCGSetMaxIter (fluid_solver.h:1873) */
+    int32_t freeze_p_matrix;          /* 1 = keep the iteration-1 p' matrix (reference values1d behaviour) */
+    int32_t precond;                  /* HS_PRECOND_* */
+    int32_t precision;                /* HS_PREC_* */
+    int32_t mg_levels;                /* multigrid levels including the finest (GMG only) */
+    int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction */
+    int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
+    int32_t mg_dense_coarse;          /* 1 = dense LU (MFMA) on the coarsest level */
+    double  t_fluid_init;             /* fluid t_temp fill, 293 (fluid_solver.h:108) */
+    double  t_solid_init;             /* solid t_temp fill, T1 = 333 (mobs_solver.h:55) */
+    double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
+    int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
+    int32_t reserved[7];
+} hs_params;
+
+typedef struct {
+    int32_t mom_sweeps[3];            /* GS sweeps run by solveX/Y/Zmomentum this iteration */
+    int32_t t_sweeps;                 /* fluid (Mode R) / conjugate (Mode C) energy sweeps */
+    int32_t ts_sweeps;                /* solid sweeps (Mode R) */
+    int32_t p_iters;                  /* p' Krylov iterations */
+    int32_t p_converged;              /* 1 if the stop rule was met before maxit */
+    int32_t pad0;
+    double  p_relres;                 /* sqrt(<r,r>/<b,b>) of the recurrence at exit */
+    double  p_true_relres;            /* ||b - A x||_2 / ||b||_2 recomputed in FP64 after the solve */
+    double  mass_imbalance;           /* ||b||_2 of the p' right-hand side (continuity residual) */
+    double  energy_residual;          /* |sum of energy cell residuals| / max(sum q V, tiny) (Mode C) */
+    double  heat_in;                  /* sum q V over solid cells (Mode C) */
+    double  mom_residual[3];          /* last GS residual value of each momentum component */
+    double  t_residual, ts_residual;
+} hs_stats;
+
+/* array identifiers for hs_get_array / hs_put_array (layouts above) */
+#define HS_A_U        0   /* u_star  U */
+#define HS_A_V        1   /* v_star  V */
+#define HS_A_W        2   /* w_star  W */
+#define HS_A_UP       3   /* up      U  (momentum diagonal, fluid_solver.h:625) */
+#define HS_A_VP       4   /* vp      V */
+#define HS_A_WP       5   /* wp      W */
+#define HS_A_P        6   /* p       C  (cell pressure, LocalCell.data) */
+#define HS_A_PTEMP    7   /* p_temp  P */
+#define HS_A_PPRIME   8   /* p_prime P */
+#define HS_A_TF       9   /* fluid t_temp P (Mode C: the conjugate temperature) */
+#define HS_A_TS      10   /* solid t_temp P (Mode R; same padded layout as the fluid) */
+#define HS_A_Q       11   /* heat source per cell, C */
+#define HS_A_K       12   /* conductivity per cell, C */
+#define HS_A_B       13   /* p' right-hand side, P */
+#define HS_A_COUNT   14
+
+typedef struct hs_ctx hs_ctx;
+
+/* ---- lifecycle ---- */
+void        hs_params_default(int32_t mode, hs_params* p);
+/* FluidSolver::FluidSolver fluid_solver.h:93-109 + MobSolver::MobSolver mobs_solver.h:50-56:
+ * copies the grid to the device, allocates the padded work arrays, fills t_temp. */
+int         hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out);
+void        hs_destroy(hs_ctx* c);
+const char* hs_strerror(int code);
+int64_t     hs_array_size(const hs_ctx* c, int32_t which);
+
+/* ---- the reference's solver methods ---- */
+/* FluidSolver::init fluid_solver.h:882-1237 + MobSolver::init mobs_solver.h:116-288:
+ * boundary conditions, interface lists, zeroed interface face velocities. */
+int hs_init(hs_ctx* c);
+/* FluidSolver::solve fluid_solver.h:1401-1633: x/y/z momentum, p' assembly + solve,
+ * velocity / pressure correction, fluid energy. */
+int hs_fluid_solve(hs_ctx* c, hs_stats* st);
+/* MobSolver::solve mobs_solver.h:303-371 (solid conduction sweeps). */
+int hs_solid_solve(hs_ctx* c, hs_stats* st);
+/* Solver::solve exchange block solver.h:62-148 (fluid <-> solid interface temperatures). */
+int hs_exchange(hs_ctx* c);
+/* FluidSolver::setForNext fluid_solver.h:1634-1682 + MobSolver::setForNext mobs_solver.h:290-301. */
+int hs_set_for_next(hs_ctx* c);
+/* One outer iteration of Solver::solve solver.h:56-154: the four calls above in order. */
+int hs_step(hs_ctx* c, hs_stats* st);
+/* n outer iterations; st may be NULL or hold n entries. */
+int hs_run(hs_ctx* c, int32_t n, hs_stats* st);
+
+/* ---- finer phases (the member functions FluidSolver::solve calls) ---- */
+/* solveXmomentum :577 / solveYmomentum :491 / solveZmomentum :407 (comp 0/1/2). */
+int hs_momentum(hs_ctx* c, int32_t comp, int32_t* sweeps, double* residual);
+/* HYPREIJMatrix_SolveCorrectionPressure :1713-1838 (assembly :1949-2096 / :2107-2194 + PCG :1789-1790). */
+int hs_pressure_correction(hs_ctx* c, hs_stats* st);
+/* velocity correction :1459-1571 + pressure update :1574-1622. */
+int hs_correct(hs_ctx* c);
+/* solveTemp :794-881 (Mode R) / conjugate energy sweeps (Mode C). */
+int hs_fluid_energy(hs_ctx* c, int32_t* sweeps, double* residual);
+
+/* ---- linear algebra on the current p' operator (tests, host drivers) ---- */
+/* y = A x on the P layout (non-fluid entries of x are ignored, of y are set to 0). */
+int hs_pressure_apply(hs_ctx* c, const double* x_host, double* y_host);
+/* solve A x = b on the P layout with the configured Krylov method; x on entry is
+ * ignored (x0 = 0 as in the reference, fluid_solver.h:1766). */
+int hs_pressure_solve(hs_ctx* c, const double* b_host, double* x_host, hs_stats* st);
+
+/* applyCellBoundaryCondition fluid_solver.h:324-405 on its own (Mode R only):
+ * boundary-cell pressure copies, ghost p_temp fill, p(0,0,0) = 0. */
+int hs_cell_bc(hs_ctx* c);
+
+/* ---- data ---- */
+int hs_get_array(hs_ctx* c, int32_t which, double* host, int64_t n);
+int hs_put_array(hs_ctx* c, int32_t which, const double* host, int64_t n);
+/* velocities on unpadded faces (u: (nx+1)*ny*nz, v: nx*(ny+1)*nz, w: nx*ny*(nz+1)),
+ * p and t on cells (t: fluid temperature in fluid cells, solid temperature in solid
+ * cells).  Any pointer may be NULL. */
+int hs_get_fields(hs_ctx* c, double* u, double* v, double* w, double* p, double* t);
+int hs_put_fields(hs_ctx* c, const double* u, const double* v, const double* w,
+                  const double* p, const double* t);
+/* Mode C per-cell heat source q (C layout) and conductivity k (C layout). */
+int hs_set_source(hs_ctx* c, const double* q);
+int hs_set_conductivity(hs_ctx* c, const double* k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSOLVER_H */
