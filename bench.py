@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""SIMPLE conjugate-heat-transfer iterations/s on one MI355X.
+
+Workload (the metric's configuration): 256^3 unit box with 27 seeded solid blocks,
+conductivity ratio 1000, lid + inlet forcing at Re = 1000, configuration physics,
+red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual.
+  value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG)
+  fp64_value = the same step with the V-cycle in FP64
+Each timed step is one full SIMPLE outer iteration (x/y/z momentum, p' assembly +
+solve, velocity / pressure correction, conjugate energy sweep, set-for-next), all
+inputs resident in HBM.
+
+roofline: the dominant kernel (one red-black colour pass of the finest multigrid
+level), timed live with HIP events on the library's stream around every launch in
+the timed region.  Algorithmic bytes per launch = 22 B per cell in FP32 (44 in FP64):
+the three face-coupling arrays and x read over all cells (16 B), ground and right-hand
+side read on the pass's colour (4 B), x written on that colour (2 B).
+
+cpu_baseline: the CPU oracle (reference algorithm: lexicographic Gauss-Seidel sweeps,
+unpreconditioned CG with HYPRE's stop rule), one core, first SIMPLE iteration of the
+same workload; the CG part is timed over a few iterations and extrapolated to the
+iteration count the same solve needs (taken from the GPU's identical CG).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SIMPLE iters/s at 256³ CHT; mixed vs FP64 GPU at 1e-8 rel p-residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None):
+    from hsolver_amd import abi, core
+    lib = core.library()
+    lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
+    lib.hs_prof_get.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    p = abi.Params.from_buffer_copy(case.params)
+    p.precision = precision
+    p.precond = precond
+    p.order = abi.HS_ORDER_RB if order is None else order
+    ctx = core.create(case.grid, p, case.q)
+    ctx.init()
+    for w in range(warmup):
+        st = ctx.step()
+        log(f"warmup {w + 1}/{warmup}: p' iters {st['p_iters']} true relres {st['p_true_relres']:.2e}")
+    if prof_kernel:
+        lib.hs_prof_set(ctx.h, prof_kernel)
+    stats = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        stats.append(ctx.step())
+    t1 = time.perf_counter()
+    prof = None
+    if prof_kernel:
+        n, ms = C.c_int64(), C.c_double()
+        lib.hs_prof_get(ctx.h, C.byref(n), C.byref(ms))
+        prof = (n.value, ms.value)
+        lib.hs_prof_set(ctx.h, 0)
+    ctx.close()
+    return (t1 - t0), stats, prof
+
+
+def cpu_sample(case, n_cg_full):
+    """Oracle, one core: first SIMPLE iteration with the CG capped at M1 and M2
+    iterations; per-CG-iteration cost from the difference, extrapolated to n_cg_full."""
+    from hsolver_amd import abi
+    from hsolver_amd.core import Context
+    olib = C.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+    abi.declare(olib, "or")
+    times = {}
+    for m in (2, 6):
+        p = abi.Params.from_buffer_copy(case.params)
+        p.order = abi.HS_ORDER_LEX
+        p.precond = abi.HS_PRECOND_NONE
+        p.p_maxit = m
+        ctx = Context(olib, "or", case.grid, p)
+        ctx.set_source(case.q)
+        ctx.init()
+        t0 = time.perf_counter()
+        ctx.step()
+        times[m] = time.perf_counter() - t0
+        ctx.close()
+        log(f"cpu oracle step with CG capped at {m}: {times[m]:.2f} s")
+    t_it = max((times[6] - times[2]) / 4.0, 0.0)
+    t_rest = max(times[2] - 2 * t_it, 0.0)
+    t_full = t_rest + n_cg_full * t_it
+    return t_full, t_rest, t_it, sum(times.values())
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="grid cells per side (the workload is 256)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 multigrid comparison run")
+    ap.add_argument("--no-cg", action="store_true", help="skip the unpreconditioned FP64 CG drop-in step")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {max(args.gpus, world)}); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from hsolver_amd import abi, configs
+    t_case = time.perf_counter()
+    case = configs.c3(args.n)
+    nx, ny, nz = case.shape
+    ncell = nx * ny * nz
+    solid_frac = float(case.grid.cell_type.mean())
+    log(f"workload {case.workload}: {ncell} cells, solid fraction {solid_frac:.3f} "
+        f"(built in {time.perf_counter() - t_case:.1f} s)")
+
+    dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
+                                       prof_kernel=1)
+    value = args.steps / dt_mixed
+    log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
+
+    fp64_value = None
+    st_fp64 = []
+    if not args.no_fp64:
+        dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup)
+        fp64_value = args.steps / dt64
+        log(f"fp64 gmg: {fp64_value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_fp64]}")
+
+    cg_step = None
+    if not args.no_cg:
+        # the reference's solver as coded (HYPRE PCG, identity preconditioner), first step
+        dtcg, stcg, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_NONE, 1, 0)
+        cg_step = {"s_per_step": dtcg, "p_iters": stcg[0]["p_iters"],
+                   "p_true_relres": stcg[0]["p_true_relres"]}
+        log(f"fp64 cg (reference solver) first step: {dtcg:.2f} s, {stcg[0]['p_iters']} iterations")
+
+    n_launch, ms_total = prof
+    avg_ms = ms_total / max(n_launch, 1)
+    bytes_per_launch = 22.0 * ncell
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_mg_smooth<float> (finest level, one red-black colour pass)",
+                "launches": n_launch, "avg_us": round(avg_ms * 1e3, 2),
+                "bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        n_cg = cg_step["p_iters"] if cg_step else 1000
+        t_full, t_rest, t_it, spent = cpu_sample(case, n_cg)
+        cpu = {"value": round(1.0 / t_full, 6), "unit": "SIMPLE iters/s", "cores": 1, "kind": "port",
+               "sample": (f"CPU oracle (lexicographic GS + unpreconditioned CG, HYPRE stop rule), first SIMPLE "
+                          f"iteration of the same {nx}x{ny}x{nz} workload: non-CG part {t_rest:.2f} s measured, "
+                          f"CG {t_it:.3f} s/iteration measured over 4 iterations and extrapolated to the "
+                          f"{n_cg} iterations that solve needs; {spent:.1f} s of CPU work"),
+               "s_per_iter": round(t_full, 2)}
+
+    relres = [s["p_true_relres"] for s in st_mixed]
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "SIMPLE iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000.0 / value, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
+                   "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG V(2,2), RB-GS",
+                   "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},
+        "fp64_value": round(fp64_value, 4) if fp64_value else None,
+        "mixed_over_fp64": round(value / fp64_value, 4) if fp64_value else None,
+        "p_iters_mixed": [s["p_iters"] for s in st_mixed],
+        "p_iters_fp64": [s["p_iters"] for s in st_fp64],
+        "p_true_relres_max": max(relres) if relres else None,
+        "mass_imbalance_last": st_mixed[-1]["mass_imbalance"] if st_mixed else None,
+        "energy_residual_last": st_mixed[-1]["energy_residual"] if st_mixed else None,
+        "fp64_cg_first_step": cg_step,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -10,8 +10,6 @@
 
 #include "hs_internal.h"
 
-int mask_fluid(hs_ctx* c, double* x);
-
 namespace {
 
 /* fluid_solver.h:1248-1261 cell widths with the reference's end fallbacks. */
@@ -221,6 +219,33 @@ int do_correct(hs_ctx* c) {
 
 }  // namespace
 
+// Records a start event if kernel is the one being timed; the pool is folded (summed
+// after a sync on its last event) when full, so any number of launches can be timed.
+int prof_begin(hs_ctx* c, int kernel) {
+    if (c->prof_kernel != kernel || c->prof_ev.empty()) return -1;
+    if (c->prof_used + 2 > (int)c->prof_ev.size()) {
+        if (hipEventSynchronize(c->prof_ev[c->prof_used - 1]) != hipSuccess) return -1;
+        for (int e = 0; e < c->prof_used; e += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->prof_ev[e], c->prof_ev[e + 1]) == hipSuccess) {
+                c->prof_ms += ms;
+                c->prof_count += 1;
+            }
+        }
+        c->prof_used = 0;
+    }
+    const int slot = c->prof_used;
+    if (hipEventRecord(c->prof_ev[slot], c->stream) != hipSuccess) return -1;
+    c->prof_used += 2;
+    return slot;
+}
+
+int prof_end(hs_ctx* c, int slot) {
+    if (slot < 0) return HS_OK;
+    HS_CHECK(hipEventRecord(c->prof_ev[slot + 1], c->stream));
+    return HS_OK;
+}
+
 extern "C" {
 
 void hs_params_default(int32_t mode, hs_params* p) {
@@ -326,7 +351,7 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     A(&c->ptemp, d.nP); A(&c->pprime, d.nP); A(&c->tf, d.nP); A(&c->ts, d.nP); A(&c->b, d.nP);
     if (modeR(c)) { A(&c->ax, d.nP); A(&c->ay, d.nP); A(&c->az, d.nP); }
     else { A(&c->ax, d.nU); A(&c->ay, d.nV); A(&c->az, d.nW); }
-    A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
+    A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
     A(&c->dres, std::max(std::max(d.nU, d.nV), std::max(d.nW, d.nP)));
     A(&c->partials, 65536);
     if (e == HS_OK) e = dalloc(&c->counter, 4);
@@ -395,12 +420,13 @@ void hs_destroy(hs_ctx* c) {
     gmg_free(c);
     double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
                       c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->ks, c->kx, c->kz,
-                      c->kq, c->dres, c->partials};
+                      c->kq, c->kb, c->dres, c->partials};
     for (double* b : bufs) if (b) hipFree(b);
     if (c->counter) hipFree(c->counter);
     if (c->cls) hipFree(c->cls);
     if (c->sctl) hipFree(c->sctl);
     if (c->wctl) hipFree(c->wctl);
+    for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     if (c->if_fluid) hipFree(c->if_fluid);
     if (c->if_solid) hipFree(c->if_solid);
     if (c->h_sctl) hipHostFree(c->h_sctl);
@@ -532,9 +558,9 @@ int hs_pressure_apply(hs_ctx* c, const double* x_host, double* y_host) {
     if (!c || !x_host || !y_host) return HS_E_ARG;
     if (!c->have_matrix) return HS_E_STATE;
     const long n = c->d.nP;
-    HS_CHECK(hipMemcpyAsync(c->kz, x_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HS_TRY(mask_fluid(c, c->kz));
-    HS_TRY(spmv(c, c->kz, c->kq));
+    HS_CHECK(hipMemcpyAsync(c->kb, x_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_TRY(mask_fluid(c, c->kb));
+    HS_TRY(spmv(c, c->kb, c->kq));
     HS_CHECK(hipMemcpyAsync(y_host, c->kq, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
     return HS_OK;
@@ -544,13 +570,13 @@ int hs_pressure_solve(hs_ctx* c, const double* b_host, double* x_host, hs_stats*
     if (!c || !b_host || !x_host) return HS_E_ARG;
     if (!c->have_matrix) return HS_E_STATE;
     const long n = c->d.nP;
-    HS_CHECK(hipMemcpyAsync(c->kz, b_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HS_TRY(mask_fluid(c, c->kz));
+    HS_CHECK(hipMemcpyAsync(c->kb, b_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
-    int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kz, c->kx, &loc) : cg_solve(c, c->kz, c->kx, &loc);
+    int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc) : cg_solve(c, c->kb, c->kx, &loc);
     if (e != HS_OK) return e;
     double rel, bn;
-    HS_TRY(true_relres(c, c->kz, c->kx, &rel, &bn));
+    HS_TRY(true_relres(c, c->kb, c->kx, &rel, &bn));
     loc.p_true_relres = rel;
     loc.mass_imbalance = bn;
     HS_CHECK(hipMemcpyAsync(x_host, c->kx, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -663,6 +689,55 @@ int hs_cell_bc(hs_ctx* c) {
     HS_TRY(cell_bc_ref(c));
     HS_CHECK(hipStreamSynchronize(c->stream));
     return HS_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+/* Live kernel timing (no counterpart in the reference, whose only timer is clock()
+ * around outer iterations, solver.h:53-54,156-158): kernel = HS_K_* starts recording
+ * HIP events on the context's stream around every launch of that kernel; 0 stops. */
+int hs_prof_set(hs_ctx* c, int32_t kernel) {
+    if (!c) return HS_E_ARG;
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    if (c->prof_ev.empty() && kernel != 0) {
+        c->prof_ev.resize(2048);
+        for (auto& e : c->prof_ev) HS_CHECK(hipEventCreate(&e));
+    }
+    c->prof_kernel = kernel;
+    c->prof_used = 0;
+    c->prof_count = 0;
+    c->prof_ms = 0.0;
+    return HS_OK;
+}
+
+/* number of timed launches and their summed duration (ms) since hs_prof_set */
+int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms) {
+    if (!c) return HS_E_ARG;
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    for (int e = 0; e < c->prof_used; e += 2) {
+        float ms = 0.f;
+        HS_CHECK(hipEventElapsedTime(&ms, c->prof_ev[e], c->prof_ev[e + 1]));
+        c->prof_ms += ms;
+        c->prof_count += 1;
+    }
+    c->prof_used = 0;
+    if (count) *count = c->prof_count;
+    if (total_ms) *total_ms = c->prof_ms;
+    return HS_OK;
+}
+
+/* active cells of each multigrid level (for byte counts); returns the level count */
+int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels) {
+    if (!c) return HS_E_ARG;
+    int n = 0;
+    for (MGLevel* m : c->mg) {
+        if (n >= max_levels) break;
+        dims[3 * n] = m->nx; dims[3 * n + 1] = m->ny; dims[3 * n + 2] = m->nz;
+        ++n;
+    }
+    return n;
 }
 
 }  // extern "C"

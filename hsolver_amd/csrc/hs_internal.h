@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -97,8 +98,15 @@ struct SweepCtl {
 constexpr int RED_THREADS = 256;
 constexpr int MAX_RED_BLOCKS = 4096;
 
-// Multigrid level (Mode C pressure / energy operators on coarser grids).
-struct MGLevel;
+// Multigrid level (Mode C pressure operator on coarser grids); arrays are float or
+// double (f32) in the padded layout (nx+2)(ny+2)(nz+2) of the level.
+struct MGLevel {
+    int level;
+    int nx, ny, nz;
+    long sy, sz, n;
+    bool f32;
+    void *cw, *cs, *cb, *g, *x, *b;
+};
 
 struct hs_ctx {
     hs_params prm;
@@ -129,6 +137,7 @@ struct hs_ctx {
     double *ax = nullptr, *ay = nullptr, *az = nullptr;
     // Krylov work (P layout)
     double *kr = nullptr, *kp = nullptr, *ks = nullptr, *kx = nullptr, *kz = nullptr, *kq = nullptr;
+    double* kb = nullptr;               // right-hand side staged from the host
     // per-point squared change for lexicographic sweeps (max layout size)
     double* dres = nullptr;
     // reductions
@@ -146,7 +155,22 @@ struct hs_ctx {
     // multigrid hierarchy (created on demand)
     std::vector<MGLevel*> mg;
     int last_iters = 0;
+
+    // live per-kernel timing: HIP events around every launch of one chosen kernel
+    int prof_kernel = 0;
+    std::vector<hipEvent_t> prof_ev;    // pairs (start, stop)
+    int prof_used = 0;                  // events recorded since the last fold
+    long prof_count = 0;
+    double prof_ms = 0.0;
 };
+
+// kernel ids for hs_prof_set
+#define HS_K_MG_SMOOTH0  1   /* one red-black colour pass on the finest multigrid level */
+#define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
+#define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
+
+int prof_begin(hs_ctx* c, int kernel);   // returns slot or -1
+int prof_end(hs_ctx* c, int slot);
 
 // ---- kernel launchers (defined in the .hip files) ----
 int launch_fill(hs_ctx* c, double* a, long n, double v);
@@ -166,6 +190,8 @@ int assemble_rhs(hs_ctx* c);
 int spmv(hs_ctx* c, const double* x, double* y);
 int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
 int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
+int cg_spmv_dot(hs_ctx* c, const double* p, double* q);
+int mask_fluid(hs_ctx* c, double* x);
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);
 int correct_velocity(hs_ctx* c);
 int update_pressure(hs_ctx* c);

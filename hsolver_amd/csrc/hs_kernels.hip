@@ -17,61 +17,20 @@
 
 #include "hs_internal.h"
 
+#include "hs_device.h"
+
 namespace {
 
-constexpr int BX = 64, BY = 4, BT = BX * BY;
-constexpr int FLAT = 256;
-constexpr int FLAT_BLOCKS = 2048;
+using hsd::BX;
+using hsd::BY;
+using hsd::BT;
+using hsd::FLAT;
+using hsd::FLAT_BLOCKS;
+using hsd::block_sum;
+using hsd::reduce_last;
+using hsd::wave_sum;
 
 __device__ __forceinline__ double posd(double a) { return (a < 0.0) ? 0.0 : a; }
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Block sum over a block of NT threads (NT multiple of 64); result valid in tid 0.
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, int tid) {
-    __shared__ double sh[NT / 64];
-    v = wave_sum(v);
-    const int lane = tid & 63, wid = tid >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (wid == 0) {
-        r = (lane < NT / 64) ? sh[lane] : 0.0;
-        r = wave_sum(r);
-    }
-    __syncthreads();
-    return r;
-}
-
-// Writes the block partial; returns true in every thread of the block that finished
-// last, with the full sum in *out (valid in tid 0).
-template <int NT>
-__device__ bool reduce_last(double v, int tid, int bid, int nblocks, double* partials,
-                            unsigned int* counter, double* out) {
-    __shared__ int am_last;
-    const double bs = block_sum<NT>(v, tid);
-    if (tid == 0) {
-        __hip_atomic_store(&partials[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double s = 0.0;
-    for (int i = tid; i < nblocks; i += NT)
-        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = block_sum<NT>(s, tid);
-    if (tid == 0) {
-        *out = s;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
-}
 
 // Gauss-Seidel loop control of solveX/Y/Zmomentum :580-658, solveTemp :797-880
 // (quirk 1: accumulator starts at 1, divided by N, never reset) and
@@ -859,16 +818,8 @@ __global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, doub
 
 // =================================================================== launchers ===
 namespace {
-inline int kchunk(int nk, long tiles) {
-    // aim for ~4096 blocks
-    long want = 4096 / (tiles > 0 ? tiles : 1);
-    if (want < 1) want = 1;
-    int kc = (int)((nk + want - 1) / want);
-    return kc < 1 ? 1 : kc;
-}
-inline dim3 grid3(int ni, int nj, int nk, int kc) {
-    return dim3((ni + BX - 1) / BX, (nj + BY - 1) / BY, (nk + kc - 1) / kc);
-}
+using hsd::kchunk;
+using hsd::grid3;
 MomArgs mom_args(hs_ctx* c, double* ap) {
     MomArgs a;
     a.d = c->d; a.g = c->g; a.u = c->u; a.v = c->v; a.w = c->w; a.ap = ap;
@@ -987,10 +938,12 @@ int momentum_sweep(hs_ctx* c, int comp) {
         const int kc = kchunk(nk, tiles);
         const dim3 gr = grid3(ni, nj, nk, kc);
         for (int colour = 0; colour < 2; ++colour) {
+            const int slot = comp == 0 ? prof_begin(c, HS_K_MOM_RB) : -1;
             if (comp == 0) k_mom_rb<0><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
             else if (comp == 1) k_mom_rb<1><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
             else k_mom_rb<2><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
             HS_CHECK(hipGetLastError());
+            HS_TRY(prof_end(c, slot));
         }
     } else {
         const long nl = comp == 0 ? d.nU : (comp == 1 ? d.nV : d.nW);
@@ -1107,7 +1060,9 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     int batch = c->last_iters > 8 ? c->last_iters - 4 : 8;
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
+            const int slot = prof_begin(c, HS_K_SPMV);
             k_cg_spmv<<<gr, dim3(BX, BY), 0, c->stream>>>(o, c->kp, c->ks, kc, c->partials, c->counter, c->sctl);
+            HS_TRY(prof_end(c, slot));
             k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, c->kp, c->ks, c->partials, c->counter, c->sctl);
             k_cg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, c->kr, c->kp, c->sctl);
         }
@@ -1176,4 +1131,16 @@ int mask_fluid(hs_ctx* c, double* x) {
     k_mask_fluid<<<1024, 256, 0, c->stream>>>(c->d.nP, c->cls, x);
     HS_CHECK(hipGetLastError());
     return HS_OK;
+}
+
+// q = A p with <p,q> and alpha = gamma / <p,q> in the last block (shared by CG and FCG).
+int cg_spmv_dot(hs_ctx* c, const double* p, double* q) {
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    const int slot = prof_begin(c, HS_K_SPMV);
+    k_cg_spmv<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(op_args(c), p, q, kc, c->partials,
+                                                                           c->counter, c->sctl);
+    HS_CHECK(hipGetLastError());
+    return prof_end(c, slot);
 }

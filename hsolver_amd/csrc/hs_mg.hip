@@ -1,6 +1,424 @@
-// hs_mg.hip — geometric multigrid preconditioner and flexible CG (filled in below).
-#include "hs_internal.h"
+// hs_mg.hip — geometric multigrid preconditioner and flexible CG for the p' operator.
+//
+// Operator on every level (cell-centred, padded layout (n+2)^3 of that level):
+//   (A x)_P = D_P x_P - cw_P x_W - cw_E x_E - cs_P x_S - cs_N x_N - cb_P x_B - cb_T x_T,
+//   D_P = g_P + cw_P + cw_E + cs_P + cs_N + cb_P + cb_T,
+// with cw/cs/cb the coupling across the west/south/bottom face of P (zero unless both
+// cells are active) and g the "ground" conductance of faces that end on a Dirichlet
+// p' = 0 ghost (walls, solids, domain boundary: the reference's convention,
+// fluid_solver.h:1996).  Inactive cells have D = 0 and x = 0.
+//
+// Coarsening (2x2x2 agglomeration): the coupling across a coarse face is the sum over
+// its four fine sub-faces of the series conductance from coarse centre to coarse centre,
+// 1 / (1/(2 c_a) + 1/c_b + 1/(2 c_c)), where c_b is the fine coupling across the coarse
+// face and c_a, c_c the fine couplings inside the two coarse cells (harmonic averaging
+// across coefficient jumps, Alcouffe-Brandt-Dendy-Painter 1981); ground halves
+// (g_c = 1/2 sum g_f).  On uniform coefficients both give exactly the rediscretised
+// operator on the coarse grid.  Restriction sums the children's residuals, prolongation
+// injects the parent's correction; smoothing is red-black Gauss-Seidel (symmetric
+// order: red, black before the coarse correction, black, red after).
+//
+// The outer solver is flexible CG (Notay 2000, one-term truncation: beta =
+// <z_new, r_new - r_old> / <z_old, r_old>) in FP64 with the stop rule of the CG path;
+// the V-cycle runs in float (HS_PREC_MIXED) or double.
+#include <cfloat>
+#include <cmath>
 
-int gmg_setup(hs_ctx* c) { (void)c; return HS_E_ARG; }
-int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) { (void)c; (void)b; (void)x; (void)st; return HS_E_ARG; }
-void gmg_free(hs_ctx* c) { (void)c; }
+#include "hs_device.h"
+
+using hsd::BT;
+using hsd::BX;
+using hsd::BY;
+using hsd::FLAT;
+using hsd::FLAT_BLOCKS;
+
+namespace {
+
+template <typename T>
+struct Lv {
+    int nx, ny, nz;
+    long sy, sz;
+    T *cw, *cs, *cb, *g, *x, *b;
+};
+
+template <typename T>
+Lv<T> lv(const MGLevel* m) {
+    return Lv<T>{m->nx, m->ny, m->nz, m->sy, m->sz, (T*)m->cw, (T*)m->cs, (T*)m->cb, (T*)m->g, (T*)m->x, (T*)m->b};
+}
+
+// fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask
+template <typename T>
+__global__ void k_mg_fine(Dims d, const uint8_t* cls, const double* ax, const double* ay, const double* az, Lv<T> L) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > d.nx) return;
+    auto fl = [&](long ii, long jj, long kk) { return cls[idxP(d, ii, jj, kk)] == CELL_FLUID; };
+    const long c = idxP(d, i, j, k);
+    const bool f = fl(i, j, k);
+    const double w = ax[idxU(d, i, j, k)], e = ax[idxU(d, i + 1, j, k)];
+    const double s = ay[idxV(d, i, j, k)], n = ay[idxV(d, i, j + 1, k)];
+    const double b = az[idxW(d, i, j, k)], t = az[idxW(d, i, j, k + 1)];
+    L.cw[c] = (T)((f && fl(i - 1, j, k)) ? w : 0.0);
+    L.cs[c] = (T)((f && fl(i, j - 1, k)) ? s : 0.0);
+    L.cb[c] = (T)((f && fl(i, j, k - 1)) ? b : 0.0);
+    double g = 0.0;
+    if (f) {
+        if (!fl(i - 1, j, k)) g += w;
+        if (!fl(i + 1, j, k)) g += e;
+        if (!fl(i, j - 1, k)) g += s;
+        if (!fl(i, j + 1, k)) g += n;
+        if (!fl(i, j, k - 1)) g += b;
+        if (!fl(i, j, k + 1)) g += t;
+    }
+    L.g[c] = (T)g;
+}
+
+template <typename T>
+__device__ __forceinline__ T series(T ca, T cb, T cc) {
+    if (!(cb > T(0))) return T(0);
+    if (!(ca > T(0))) ca = cb;
+    if (!(cc > T(0))) cc = cb;
+    return T(1) / (T(0.5) / ca + T(1) / cb + T(0.5) / cc);
+}
+
+template <typename T>
+__global__ void k_mg_coarsen(Lv<T> F, Lv<T> C) {
+    const int I = 1 + blockIdx.x * blockDim.x + threadIdx.x, J = 1 + blockIdx.y, K = 1 + blockIdx.z;
+    if (I > C.nx) return;
+    const long cc = I + C.sy * J + C.sz * K;
+    auto fi = [&](long i, long j, long k) { return i + F.sy * j + F.sz * k; };
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1, k0 = 2 * K - 1;
+    const int ni = (2 * I <= F.nx) ? 2 : 1, nj = (2 * J <= F.ny) ? 2 : 1, nk = (2 * K <= F.nz) ? 2 : 1;
+    T sw = 0, ss = 0, sb = 0, g = 0;
+    for (int dk = 0; dk < nk; ++dk)
+        for (int dj = 0; dj < nj; ++dj) {
+            if (I >= 2) {
+                const long f = fi(i0, j0 + dj, k0 + dk);
+                const T cc2 = (ni == 2) ? F.cw[f + 1] : T(0);
+                sw += series<T>(F.cw[f - 1], F.cw[f], cc2);
+            }
+        }
+    for (int dk = 0; dk < nk; ++dk)
+        for (int di = 0; di < ni; ++di) {
+            if (J >= 2) {
+                const long f = fi(i0 + di, j0, k0 + dk);
+                const T cc2 = (nj == 2) ? F.cs[f + F.sy] : T(0);
+                ss += series<T>(F.cs[f - F.sy], F.cs[f], cc2);
+            }
+        }
+    for (int dj = 0; dj < nj; ++dj)
+        for (int di = 0; di < ni; ++di) {
+            if (K >= 2) {
+                const long f = fi(i0 + di, j0 + dj, k0);
+                const T cc2 = (nk == 2) ? F.cb[f + F.sz] : T(0);
+                sb += series<T>(F.cb[f - F.sz], F.cb[f], cc2);
+            }
+        }
+    for (int dk = 0; dk < nk; ++dk)
+        for (int dj = 0; dj < nj; ++dj)
+            for (int di = 0; di < ni; ++di) g += F.g[fi(i0 + di, j0 + dj, k0 + dk)];
+    C.cw[cc] = sw;
+    C.cs[cc] = ss;
+    C.cb[cc] = sb;
+    C.g[cc] = T(0.5) * g;
+}
+
+template <typename T>
+__device__ __forceinline__ T diag_of(const Lv<T>& L, long c) {
+    return L.g[c] + L.cw[c] + L.cw[c + 1] + L.cs[c] + L.cs[c + L.sy] + L.cb[c] + L.cb[c + L.sz];
+}
+
+template <typename T>
+__device__ __forceinline__ T offsum(const Lv<T>& L, long c) {
+    const T* x = L.x;
+    return L.cw[c] * x[c - 1] + L.cw[c + 1] * x[c + 1] + L.cs[c] * x[c - L.sy] + L.cs[c + L.sy] * x[c + L.sy] +
+           L.cb[c] * x[c - L.sz] + L.cb[c + L.sz] * x[c + L.sz];
+}
+
+// one colour of red-black Gauss-Seidel
+template <typename T>
+__global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > L.nx || j > L.ny) return;
+    const int k0 = 1 + blockIdx.z * kc, k1 = min(L.nz, k0 + kc - 1);
+    for (int k = k0; k <= k1; ++k) {
+        if (((i + j + k) & 1) != colour) continue;
+        const long c = i + L.sy * j + L.sz * k;
+        const T D = diag_of(L, c);
+        if (!(D > T(0))) continue;
+        L.x[c] = (L.b[c] + offsum(L, c)) / D;
+    }
+}
+
+// b_c = sum over active children of (b - A x); x_c = 0
+template <typename T>
+__global__ void k_mg_restrict(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    const int I = 1 + blockIdx.x * blockDim.x + threadIdx.x, J = 1 + blockIdx.y, K = 1 + blockIdx.z;
+    if (I > C.nx) return;
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1, k0 = 2 * K - 1;
+    const int ni = (2 * I <= F.nx) ? 2 : 1, nj = (2 * J <= F.ny) ? 2 : 1, nk = (2 * K <= F.nz) ? 2 : 1;
+    T r = 0;
+    for (int dk = 0; dk < nk; ++dk)
+        for (int dj = 0; dj < nj; ++dj)
+            for (int di = 0; di < ni; ++di) {
+                const long f = (i0 + di) + F.sy * (j0 + dj) + F.sz * (k0 + dk);
+                const T D = diag_of(F, f);
+                if (!(D > T(0))) continue;
+                r += F.b[f] - (D * F.x[f] - offsum(F, f));
+            }
+    const long cc = I + C.sy * J + C.sz * K;
+    C.b[cc] = r;
+    C.x[cc] = T(0);
+}
+
+// x_f += x_c(parent) on active fine cells
+template <typename T>
+__global__ void k_mg_prolong(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > F.nx) return;
+    const long f = i + F.sy * j + F.sz * k;
+    if (!(diag_of(F, f) > T(0))) return;
+    F.x[f] += C.x[((i + 1) >> 1) + C.sy * ((j + 1) >> 1) + C.sz * ((k + 1) >> 1)];
+}
+
+template <typename T>
+__global__ void k_mg_load(long n, const double* r, T* b, T* x, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        b[t] = (T)r[t];
+        x[t] = T(0);
+    }
+}
+
+template <typename T>
+__global__ void k_mg_store(long n, const T* x, double* z, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) z[t] = (double)x[t];
+}
+
+// ---- flexible CG pieces (FP64) ----
+// p = z, gamma = <r, z>
+__global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const double* z, double* p,
+                                                    double* partials, unsigned* counter, SolveCtl* ctl) {
+    if (ctl->done) return;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        p[t] = z[t];
+        acc += r[t] * z[t];
+    }
+    double tot;
+    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0)
+        ctl->gamma = tot;
+}
+
+// x += alpha p, r -= alpha q, <r,r>; stop rule
+__global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const double* p, const double* q,
+                                                     double* partials, unsigned* counter, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const double alpha = ctl->alpha;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        x[t] += alpha * p[t];
+        const double rv = r[t] - alpha * q[t];
+        r[t] = rv;
+        acc += rv * rv;
+    }
+    double tot;
+    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
+        ctl->iters += 1;
+        ctl->rr = tot;
+        if (tot / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; return; }
+        if (ctl->iters >= ctl->maxit) { ctl->done = 1; return; }
+    }
+}
+
+// gamma_new = <z, r>, beta = -alpha <z, q> / gamma_old
+__global__ __launch_bounds__(FLAT) void k_fcg_beta(long n, const double* z, const double* r, const double* q,
+                                                   double* partials, unsigned* counter, SolveCtl* ctl) {
+    if (ctl->done) return;
+    double a = 0.0, b = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        a += z[t] * r[t];
+        b += z[t] * q[t];
+    }
+    double zr, zq;
+    if (hsd::reduce2_last<FLAT>(a, b, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &zr, &zq) &&
+        threadIdx.x == 0) {
+        ctl->beta = -ctl->alpha * zq / ctl->gamma;
+        ctl->gamma_old = ctl->gamma;
+        ctl->gamma = zr;
+    }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(FLAT) void k_fcg_pupdate(long n, const double* z, double* p, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    const double beta = ctl->beta;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) p[t] = z[t] + beta * p[t];
+}
+
+__global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, double* x, double* r, double* partials,
+                                                    unsigned* counter, SolveCtl* ctl) {
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+        const double bv = b[t];
+        x[t] = 0.0;
+        r[t] = bv;
+        acc += bv * bv;
+    }
+    double tot;
+    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
+        ctl->bb = tot;
+        ctl->rr = tot;
+        ctl->iters = 0;
+        ctl->converged = (tot == 0.0) ? 1 : 0;
+        ctl->done = (tot == 0.0) ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------ host side ---
+template <typename T>
+int alloc_level(MGLevel* m) {
+    void** arrs[6] = {&m->cw, &m->cs, &m->cb, &m->g, &m->x, &m->b};
+    for (void** a : arrs) {
+        if (hipMalloc(a, m->n * sizeof(T)) != hipSuccess) return HS_E_NOMEM;
+        if (hipMemset(*a, 0, m->n * sizeof(T)) != hipSuccess) return HS_E_HIP;
+    }
+    return HS_OK;
+}
+
+template <typename T>
+int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
+    Lv<T> L = lv<T>(m);
+    const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
+    const int kc = hsd::kchunk(L.nz, tiles);
+    const dim3 gr = hsd::grid3(L.nx, L.ny, L.nz, kc);
+    for (int s = 0; s < sweeps; ++s)
+        for (int h = 0; h < 2; ++h) {
+            const int colour = forward ? h : 1 - h;
+            const int slot = m->level == 0 ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
+            k_mg_smooth<T><<<gr, dim3(BX, BY), 0, c->stream>>>(L, kc, colour, c->sctl);
+            HS_TRY(prof_end(c, slot));
+        }
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+template <typename T>
+int vcycle(hs_ctx* c, const double* r, double* z) {
+    const int nl = (int)c->mg.size();
+    MGLevel* m0 = c->mg[0];
+    k_mg_load<T><<<1024, 256, 0, c->stream>>>(m0->n, r, (T*)m0->b, (T*)m0->x, c->sctl);
+    HS_CHECK(hipGetLastError());
+    for (int l = 0; l + 1 < nl; ++l) {
+        HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
+        Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
+        k_mg_restrict<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, c->sctl);
+        HS_CHECK(hipGetLastError());
+    }
+    HS_TRY(smooth<T>(c, c->mg[nl - 1], c->prm.mg_coarse_sweeps, true));
+    HS_TRY(smooth<T>(c, c->mg[nl - 1], c->prm.mg_coarse_sweeps, false));
+    for (int l = nl - 2; l >= 0; --l) {
+        Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
+        k_mg_prolong<T><<<dim3((F.nx + 63) / 64, F.ny, F.nz), 64, 0, c->stream>>>(F, C, c->sctl);
+        HS_CHECK(hipGetLastError());
+        HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
+    }
+    k_mg_store<T><<<1024, 256, 0, c->stream>>>(m0->n, (const T*)m0->x, z, c->sctl);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+template <typename T>
+int build_hierarchy(hs_ctx* c) {
+    const Dims& d = c->d;
+    if (c->mg.empty()) {
+        int nx = d.nx, ny = d.ny, nz = d.nz;
+        const int maxl = c->prm.mg_levels > 0 ? c->prm.mg_levels : 64;
+        for (int l = 0; l < maxl; ++l) {
+            MGLevel* m = new MGLevel();
+            m->level = l;
+            m->nx = nx; m->ny = ny; m->nz = nz;
+            m->sy = nx + 2; m->sz = (long)(nx + 2) * (ny + 2); m->n = m->sz * (nz + 2);
+            m->f32 = sizeof(T) == 4;
+            m->cw = m->cs = m->cb = m->g = m->x = m->b = nullptr;
+            c->mg.push_back(m);
+            HS_TRY(alloc_level<T>(m));
+            if (std::min(nx, std::min(ny, nz)) <= 4) break;
+            nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+        }
+    }
+    Lv<T> L0 = lv<T>(c->mg[0]);
+    k_mg_fine<T><<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->ax, c->ay, c->az, L0);
+    HS_CHECK(hipGetLastError());
+    for (size_t l = 0; l + 1 < c->mg.size(); ++l) {
+        Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
+        k_mg_coarsen<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C);
+        HS_CHECK(hipGetLastError());
+    }
+    return HS_OK;
+}
+
+template <typename T>
+int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+    const long n = c->d.nP;
+    HS_TRY(build_hierarchy<T>(c));
+    SolveCtl* h = c->h_sctl;
+    h->eps = c->prm.p_tol * c->prm.p_tol;
+    h->maxit = c->prm.p_maxit;
+    HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    double *r = c->kr, *p = c->kp, *q = c->ks, *z = c->kz;
+    k_fcg_setup<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, b, x, r, c->partials, c->counter, c->sctl);
+    HS_TRY(vcycle<T>(c, r, z));
+    k_fcg_start<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, r, z, p, c->partials, c->counter, c->sctl);
+    HS_CHECK(hipGetLastError());
+    int launched = 0;
+    int batch = c->last_iters > 2 ? c->last_iters - 1 : 4;
+    while (true) {
+        for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
+            HS_TRY(cg_spmv_dot(c, p, q));
+            k_fcg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, r, p, q, c->partials, c->counter, c->sctl);
+            HS_TRY(vcycle<T>(c, r, z));
+            k_fcg_beta<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, z, r, q, c->partials, c->counter, c->sctl);
+            k_fcg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, z, p, c->sctl);
+        }
+        HS_CHECK(hipGetLastError());
+        HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
+        HS_CHECK(hipStreamSynchronize(c->stream));
+        if (h->done || launched >= c->prm.p_maxit) break;
+        batch = 2;
+    }
+    c->last_iters = h->iters;
+    if (st) {
+        st->p_iters = h->iters;
+        st->p_converged = h->converged;
+        st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
+    }
+    if (!std::isfinite(h->rr)) return HS_E_NAN;
+    return HS_OK;
+}
+
+}  // namespace
+
+int gmg_setup(hs_ctx* c) {
+    if (c->prm.mode != HS_MODE_CONFIG) return HS_E_ARG;
+    return c->prm.precision == HS_PREC_MIXED ? build_hierarchy<float>(c) : build_hierarchy<double>(c);
+}
+
+int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+    // the multigrid operator form needs the symmetric face-coefficient p' operator
+    if (c->prm.mode != HS_MODE_CONFIG) return HS_E_ARG;
+    return c->prm.precision == HS_PREC_MIXED ? fcg<float>(c, b, x, st) : fcg<double>(c, b, x, st);
+}
+
+void gmg_free(hs_ctx* c) {
+    for (MGLevel* m : c->mg) {
+        void* arrs[6] = {m->cw, m->cs, m->cb, m->g, m->x, m->b};
+        for (void* a : arrs) if (a) hipFree(a);
+        delete m;
+    }
+    c->mg.clear();
+}

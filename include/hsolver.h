@@ -200,6 +200,18 @@ int hs_put_fields(hs_ctx* c, const double* u, const double* v, const double* w,
 int hs_set_source(hs_ctx* c, const double* q);
 int hs_set_conductivity(hs_ctx* c, const double* k);
 
+/* ---- live kernel timing (no counterpart in the reference; its only timer is clock()
+ *      around outer iterations, solver.h:53-54,156-158) ---- */
+#define HS_K_MG_SMOOTH0  1   /* one red-black colour pass on the finest multigrid level */
+#define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
+#define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
+/* record HIP events on the context's stream around every launch of one kernel (0: off) */
+int hs_prof_set(hs_ctx* c, int32_t kernel);
+/* launches timed and their summed duration in ms since hs_prof_set */
+int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms);
+/* multigrid level dimensions (3 ints per level); returns the number of levels */
+int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels);
+
 #ifdef __cplusplus
 }
 #endif
