@@ -92,7 +92,7 @@ def params_default(mode: int) -> Params:
     p.freeze_p_matrix = 1
     p.precond = HS_PRECOND_NONE
     p.precision = HS_PREC_FP64
-    p.mg_levels = 4
+    p.mg_levels = 0
     p.mg_pre = 2
     p.mg_post = 2
     p.mg_coarse_sweeps = 40

@@ -268,7 +268,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->freeze_p_matrix = 1;
     p->precond = HS_PRECOND_NONE;
     p->precision = HS_PREC_FP64;
-    p->mg_levels = 4;
+    p->mg_levels = 0;   // 0: coarsen until the smallest dimension is <= 4
     p->mg_pre = 2;
     p->mg_post = 2;
     p->mg_coarse_sweeps = 40;
@@ -292,6 +292,14 @@ void hs_params_default(int32_t mode, hs_params* p) {
         p->u_lid = 0.0;
         p->u_in = 1.0;
     }
+}
+
+int hs_abi_sizes(int32_t* sizes3) {
+    if (!sizes3) return HS_E_ARG;
+    sizes3[0] = (int32_t)sizeof(hs_grid);
+    sizes3[1] = (int32_t)sizeof(hs_params);
+    sizes3[2] = (int32_t)sizeof(hs_stats);
+    return HS_OK;
 }
 
 const char* hs_strerror(int code) {

@@ -12,10 +12,11 @@
 // its four fine sub-faces of the series conductance from coarse centre to coarse centre,
 // 1 / (1/(2 c_a) + 1/c_b + 1/(2 c_c)), where c_b is the fine coupling across the coarse
 // face and c_a, c_c the fine couplings inside the two coarse cells (harmonic averaging
-// across coefficient jumps, Alcouffe-Brandt-Dendy-Painter 1981); ground halves
-// (g_c = 1/2 sum g_f).  On uniform coefficients both give exactly the rediscretised
-// operator on the coarse grid.  Restriction sums the children's residuals, prolongation
-// injects the parent's correction; smoothing is red-black Gauss-Seidel (symmetric
+// across coefficient jumps, Alcouffe-Brandt-Dendy-Painter 1981); grounds add with the
+// factor that keeps the Dirichlet point where the finest level puts it (see
+// build_hierarchy).  On uniform coefficients both give exactly the rediscretised
+// operator on the coarse grid, boundary rows included.  Restriction sums the children's residuals, prolongation
+// is trilinear (see prolong_value); smoothing is red-black Gauss-Seidel (symmetric
 // order: red, black before the coarse correction, black, red after).
 //
 // The outer solver is flexible CG (Notay 2000, one-term truncation: beta =
@@ -81,7 +82,7 @@ __device__ __forceinline__ T series(T ca, T cb, T cc) {
 }
 
 template <typename T>
-__global__ void k_mg_coarsen(Lv<T> F, Lv<T> C) {
+__global__ void k_mg_coarsen(Lv<T> F, Lv<T> C, T gscale) {
     const int I = 1 + blockIdx.x * blockDim.x + threadIdx.x, J = 1 + blockIdx.y, K = 1 + blockIdx.z;
     if (I > C.nx) return;
     const long cc = I + C.sy * J + C.sz * K;
@@ -119,7 +120,7 @@ __global__ void k_mg_coarsen(Lv<T> F, Lv<T> C) {
     C.cw[cc] = sw;
     C.cs[cc] = ss;
     C.cb[cc] = sb;
-    C.g[cc] = T(0.5) * g;
+    C.g[cc] = gscale * g;
 }
 
 template <typename T>
@@ -172,7 +173,26 @@ __global__ void k_mg_restrict(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
     C.x[cc] = T(0);
 }
 
-// x_f += x_c(parent) on active fine cells
+// Trilinear interpolation of the coarse correction to fine cell (i,j,k) (1-based): the
+// fine centre sits a quarter coarse cell from its parent's centre, so each axis weighs
+// the parent 3/4 and the coarse neighbour on the fine cell's side 1/4 (27/64, 9/64,
+// 3/64, 1/64 in 3-D).  Ghost and inactive coarse cells hold 0, which is the p' = 0
+// Dirichlet value of walls and solids.  With summed (constant) restriction the
+// transfer orders satisfy m_P + m_R > 2m for this second-order operator.
+template <typename T>
+__device__ __forceinline__ T prolong_value(const Lv<T>& C, int i, int j, int k) {
+    const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
+    const int di = (i & 1) ? -1 : 1, dj = (j & 1) ? -1 : 1, dk = (k & 1) ? -1 : 1;
+    const long c0 = I + C.sy * J + C.sz * K;
+    const long ox = di, oy = dj * C.sy, oz = dk * C.sz;
+    const T* x = C.x;
+    const T a = T(0.75), b = T(0.25);
+    const T zlo = a * (a * x[c0] + b * x[c0 + ox]) + b * (a * x[c0 + oy] + b * x[c0 + ox + oy]);
+    const T zhi = a * (a * x[c0 + oz] + b * x[c0 + ox + oz]) + b * (a * x[c0 + oy + oz] + b * x[c0 + ox + oy + oz]);
+    return a * zlo + b * zhi;
+}
+
+// x_f += P x_c on active fine cells
 template <typename T>
 __global__ void k_mg_prolong(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
     if (ctl->done) return;
@@ -180,7 +200,85 @@ __global__ void k_mg_prolong(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
     if (i > F.nx) return;
     const long f = i + F.sy * j + F.sz * k;
     if (!(diag_of(F, f) > T(0))) return;
-    F.x[f] += C.x[((i + 1) >> 1) + C.sy * ((j + 1) >> 1) + C.sz * ((k + 1) >> 1)];
+    F.x[f] += prolong_value(C, i, j, k);
+}
+
+// ---- single-workgroup tail: every level from `t` down, in one launch ----
+// Small levels are launch-bound as separate kernels; one workgroup runs their
+// pre-smoothing, restriction, coarsest sweeps, prolongation and post-smoothing with
+// workgroup barriers between stages (the data stays in L2).
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_MAX_CELLS = 4096;   // levels at or below 16^3
+constexpr int MAX_LEVELS = 16;
+
+template <typename T>
+struct LvPack {
+    Lv<T> l[MAX_LEVELS];
+};
+
+template <typename T>
+__device__ void tail_smooth(const Lv<T>& L, int colour) {
+    const long ncol = (long)L.nx * L.ny * L.nz;
+    for (long t = threadIdx.x; t < ncol; t += TAIL_THREADS) {
+        const int i = 1 + (int)(t % L.nx), j = 1 + (int)((t / L.nx) % L.ny), k = 1 + (int)(t / ((long)L.nx * L.ny));
+        if (((i + j + k) & 1) != colour) continue;
+        const long c = i + L.sy * j + L.sz * k;
+        const T D = diag_of(L, c);
+        if (!(D > T(0))) continue;
+        L.x[c] = (L.b[c] + offsum(L, c)) / D;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void tail_restrict(const Lv<T>& F, const Lv<T>& C) {
+    const long nc = (long)C.nx * C.ny * C.nz;
+    for (long t = threadIdx.x; t < nc; t += TAIL_THREADS) {
+        const int I = 1 + (int)(t % C.nx), J = 1 + (int)((t / C.nx) % C.ny), K = 1 + (int)(t / ((long)C.nx * C.ny));
+        const int i0 = 2 * I - 1, j0 = 2 * J - 1, k0 = 2 * K - 1;
+        const int ni = (2 * I <= F.nx) ? 2 : 1, nj = (2 * J <= F.ny) ? 2 : 1, nk = (2 * K <= F.nz) ? 2 : 1;
+        T r = 0;
+        for (int dk = 0; dk < nk; ++dk)
+            for (int dj = 0; dj < nj; ++dj)
+                for (int di = 0; di < ni; ++di) {
+                    const long f = (i0 + di) + F.sy * (j0 + dj) + F.sz * (k0 + dk);
+                    const T D = diag_of(F, f);
+                    if (!(D > T(0))) continue;
+                    r += F.b[f] - (D * F.x[f] - offsum(F, f));
+                }
+        const long cc = I + C.sy * J + C.sz * K;
+        C.b[cc] = r;
+        C.x[cc] = T(0);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void tail_prolong(const Lv<T>& F, const Lv<T>& C) {
+    const long nf = (long)F.nx * F.ny * F.nz;
+    for (long t = threadIdx.x; t < nf; t += TAIL_THREADS) {
+        const int i = 1 + (int)(t % F.nx), j = 1 + (int)((t / F.nx) % F.ny), k = 1 + (int)(t / ((long)F.nx * F.ny));
+        const long f = i + F.sy * j + F.sz * k;
+        if (!(diag_of(F, f) > T(0))) continue;
+        F.x[f] += prolong_value(C, i, j, k);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(LvPack<T> P, int t, int nl, int pre, int post,
+                                                           int coarse, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    for (int l = t; l + 1 < nl; ++l) {
+        for (int s = 0; s < pre; ++s) { tail_smooth(P.l[l], 0); tail_smooth(P.l[l], 1); }
+        tail_restrict(P.l[l], P.l[l + 1]);
+    }
+    for (int s = 0; s < coarse; ++s) { tail_smooth(P.l[nl - 1], 0); tail_smooth(P.l[nl - 1], 1); }
+    for (int s = 0; s < coarse; ++s) { tail_smooth(P.l[nl - 1], 1); tail_smooth(P.l[nl - 1], 0); }
+    for (int l = nl - 2; l >= t; --l) {
+        tail_prolong(P.l[l], P.l[l + 1]);
+        for (int s = 0; s < post; ++s) { tail_smooth(P.l[l], 1); tail_smooth(P.l[l], 0); }
+    }
 }
 
 template <typename T>
@@ -312,15 +410,22 @@ int vcycle(hs_ctx* c, const double* r, double* z) {
     MGLevel* m0 = c->mg[0];
     k_mg_load<T><<<1024, 256, 0, c->stream>>>(m0->n, r, (T*)m0->b, (T*)m0->x, c->sctl);
     HS_CHECK(hipGetLastError());
-    for (int l = 0; l + 1 < nl; ++l) {
+    // first level small enough for the single-workgroup tail
+    int t = 0;
+    while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
+    if (t >= nl) t = nl - 1;
+    for (int l = 0; l < t; ++l) {
         HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
         k_mg_restrict<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, c->sctl);
         HS_CHECK(hipGetLastError());
     }
-    HS_TRY(smooth<T>(c, c->mg[nl - 1], c->prm.mg_coarse_sweeps, true));
-    HS_TRY(smooth<T>(c, c->mg[nl - 1], c->prm.mg_coarse_sweeps, false));
-    for (int l = nl - 2; l >= 0; --l) {
+    LvPack<T> P;
+    for (int l = 0; l < nl && l < MAX_LEVELS; ++l) P.l[l] = lv<T>(c->mg[l]);
+    k_mg_tail<T><<<1, TAIL_THREADS, 0, c->stream>>>(P, t, nl, c->prm.mg_pre, c->prm.mg_post,
+                                                    c->prm.mg_coarse_sweeps, c->sctl);
+    HS_CHECK(hipGetLastError());
+    for (int l = t - 1; l >= 0; --l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
         k_mg_prolong<T><<<dim3((F.nx + 63) / 64, F.ny, F.nz), 64, 0, c->stream>>>(F, C, c->sctl);
         HS_CHECK(hipGetLastError());
@@ -336,7 +441,7 @@ int build_hierarchy(hs_ctx* c) {
     const Dims& d = c->d;
     if (c->mg.empty()) {
         int nx = d.nx, ny = d.ny, nz = d.nz;
-        const int maxl = c->prm.mg_levels > 0 ? c->prm.mg_levels : 64;
+        const int maxl = std::min(MAX_LEVELS, c->prm.mg_levels > 0 ? c->prm.mg_levels : MAX_LEVELS);
         for (int l = 0; l < maxl; ++l) {
             MGLevel* m = new MGLevel();
             m->level = l;
@@ -355,7 +460,12 @@ int build_hierarchy(hs_ctx* c) {
     HS_CHECK(hipGetLastError());
     for (size_t l = 0; l + 1 < c->mg.size(); ++l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-        k_mg_coarsen<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C);
+        // ground conductance: the Dirichlet point sits half a finest-level cell outside the
+        // boundary face, i.e. (h_l + h_0)/2 from a level-l boundary centre, so a coarse
+        // ground is (2^l + 1)/(2^(l+1) + 1) of the summed fine grounds (2/3, 3/5, 5/9, ...)
+        const double pw = std::ldexp(1.0, (int)l);
+        const T gscale = (T)((pw + 1.0) / (2.0 * pw + 1.0));
+        k_mg_coarsen<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, gscale);
         HS_CHECK(hipGetLastError());
     }
     return HS_OK;

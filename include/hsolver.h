@@ -93,7 +93,7 @@ typedef struct {
     int32_t freeze_p_matrix;          /* 1 = keep the iteration-1 p' matrix (reference values1d behaviour) */
     int32_t precond;                  /* HS_PRECOND_* */
     int32_t precision;                /* HS_PREC_* */
-    int32_t mg_levels;                /* multigrid levels including the finest (GMG only) */
+    int32_t mg_levels;                /* multigrid levels including the finest; 0 = down to <= 4 cells a side */
     int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
     int32_t mg_dense_coarse;          /* 1 = dense LU (MFMA) on the coarsest level */
@@ -146,6 +146,8 @@ void        hs_params_default(int32_t mode, hs_params* p);
 int         hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out);
 void        hs_destroy(hs_ctx* c);
 const char* hs_strerror(int code);
+/* sizeof(hs_grid), sizeof(hs_params), sizeof(hs_stats): lets bindings check their layouts */
+int         hs_abi_sizes(int32_t* sizes3);
 int64_t     hs_array_size(const hs_ctx* c, int32_t which);
 
 /* ---- the reference's solver methods ---- */
