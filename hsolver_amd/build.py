@@ -16,6 +16,7 @@ PKG = Path(__file__).resolve().parent
 SRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libhsolver_amd.so"
+EXE = PKG / "hsolver_main"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HS_OFFLOAD_ARCH", "gfx950")
 
@@ -47,8 +48,14 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host driver (main.cpp shape) linked against the library, found next to it at run time
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(PKG / "host" / "hs_main.cpp"),
+           f"-L{PKG}", "-lhsolver_amd", "-Wl,-rpath,$ORIGIN", "-o", str(EXE)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {EXE}")
     return LIB
 
 

@@ -1,0 +1,72 @@
+// hs_main.cpp — host driver with the shape of the reference's main.cpp:10-161 on one
+// rank: read the `.in` mesh, build the fluid and solid solvers, run Solver::solve,
+// write result.vtk.
+//
+//   hsolver_main <file.in> [iterations=100] [out.vtk=result.vtk] [--order lex|rb]
+//
+// The physics is the reference's (HS_MODE_REF, every quirk kept); decomposition is
+// collapsed to one rank and one GPU, so the processor files of the decomposer are not
+// needed.  Exit code 0 on success, 1 on usage / input errors, 2 on solver errors.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "hs_solver.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s <file.in> [iterations] [out.vtk] [--order lex|rb]\n", argv[0]);
+        return 1;
+    }
+    const char* in = argv[1];
+    int iters = 100;
+    std::string out = "result.vtk";
+    int order = HS_ORDER_LEX;
+    int pos = 0;
+    for (int a = 2; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--order") && a + 1 < argc) {
+            order = !std::strcmp(argv[++a], "rb") ? HS_ORDER_RB : HS_ORDER_LEX;
+        } else if (pos == 0) { iters = std::atoi(argv[a]); ++pos; }
+        else if (pos == 1) { out = argv[a]; ++pos; }
+    }
+    hs_mesh* mesh = nullptr;
+    int rc = hs_mesh_read(in, &mesh);
+    if (rc != HS_OK) {
+        std::fprintf(stderr, "cannot read %s: %s\n", in, hs_strerror(rc));
+        return 1;
+    }
+    int32_t nx, ny, nz, nmobs, nwalls, nfr, nsr;
+    hs_mesh_info(mesh, &nx, &ny, &nz, &nmobs, &nwalls, &nfr, &nsr);
+    std::printf("Info: mesh %dx%dx%d, %d mobs boxes, %d fluid / %d solid regions\n", nx, ny, nz, nmobs, nfr, nsr);
+    hs_grid g;
+    hs_mesh_grid(mesh, &g);
+    hs_params p;
+    hs_params_default(HS_MODE_REF, &p);
+    p.order = order;
+    try {
+        hsolver::Context ctx(g, p);
+        hsolver::FluidSolver fluid(ctx);
+        hsolver::MobSolver mob(ctx);
+        hsolver::Solver solver;
+        solver.addFluids(&fluid);
+        if (nsr > 0) solver.addMobs(&mob);
+        const int warm = iters > 10 ? 10 : 0;
+        solver.solve(iters, warm);
+        const hs_stats& last = solver.history.back();
+        std::printf("Info: %d iterations; last: p' CG %d iters (relres %.3e), sweeps u/v/w %d/%d/%d, T %d, solid %d\n",
+                    iters, last.p_iters, last.p_relres, last.mom_sweeps[0], last.mom_sweeps[1], last.mom_sweeps[2],
+                    last.t_sweeps, last.ts_sweeps);
+        if (iters > warm)
+            std::printf("Info: wall time %.3f s over iterations %d-%d, average %.4f s\n", solver.seconds_after_warmup,
+                        warm, iters - 1, solver.seconds_after_warmup / (iters - warm));
+        hsolver::write_vtk(out.c_str(), ctx.get(), g);
+        std::printf("VTK file written: %s\n", out.c_str());
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        hs_mesh_free(mesh);
+        return 2;
+    }
+    hs_mesh_free(mesh);
+    return 0;
+}
