@@ -73,6 +73,20 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
     return (t1 - t0), stats, prof
 
 
+def pmc_traffic(n):
+    """HBM bytes per launch of the roofline kernel, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a plain
+    512 MiB device copy).  Counters cannot be read inside a timed run, so the figure is the
+    profiled one; None for other grid sizes."""
+    f = ROOT / "profiles" / "k_mg_smooth_float_c3_256.md"
+    if n != 256 or not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_sample(case, n_cg_full):
     """Oracle, one core: first SIMPLE iteration with the CG capped at M1 and M2
     iterations; per-CG-iteration cost from the difference, extrapolated to n_cg_full."""
@@ -109,6 +123,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 multigrid comparison run")
     ap.add_argument("--no-cg", action="store_true", help="skip the unpreconditioned FP64 CG drop-in step")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -117,6 +133,14 @@ def main() -> int:
         return 2
 
     from hsolver_amd import abi, configs
+    if args.copy_calibration:
+        from hsolver_amd import core
+        lib = core.library()
+        lib.hs_copy_bench.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+        ms = C.c_double()
+        rc = lib.hs_copy_bench(args.copy_calibration, 3, C.byref(ms))
+        log(f"copy calibration: {args.copy_calibration} B in {ms.value:.3f} ms "
+            f"({2 * args.copy_calibration / ms.value / 1e6:.0f} GB/s read+write), rc {rc}")
     t_case = time.perf_counter()
     case = configs.c3(args.n)
     nx, ny, nz = case.shape
@@ -150,7 +174,7 @@ def main() -> int:
     bytes_per_launch = 22.0 * ncell
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.n),
                 "kernel": "k_mg_smooth<float> (finest level, one red-black colour pass)",
                 "launches": n_launch, "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": bytes_per_launch}

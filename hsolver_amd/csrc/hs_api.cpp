@@ -362,7 +362,6 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
     A(&c->dres, std::max(std::max(d.nU, d.nV), std::max(d.nW, d.nP)));
     A(&c->partials, 65536);
-    if (e == HS_OK) e = dalloc(&c->counter, 4);
     if (e == HS_OK) e = dalloc(&c->cls, (size_t)d.nP);
     if (e == HS_OK) e = dalloc(&c->sctl, 1);
     if (e == HS_OK) e = dalloc(&c->wctl, 1);
@@ -430,7 +429,6 @@ void hs_destroy(hs_ctx* c) {
                       c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->ks, c->kx, c->kz,
                       c->kq, c->kb, c->dres, c->partials};
     for (double* b : bufs) if (b) hipFree(b);
-    if (c->counter) hipFree(c->counter);
     if (c->cls) hipFree(c->cls);
     if (c->sctl) hipFree(c->sctl);
     if (c->wctl) hipFree(c->wctl);
@@ -746,6 +744,44 @@ int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels) {
         ++n;
     }
     return n;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+/* Plain device copy of nbytes, reps times, on a private stream: the known-size
+ * reference that hardware byte counters (FETCH_SIZE / WRITE_SIZE) are scaled by.
+ * Returns the average copy time in ms. */
+int hs_copy_bench(int64_t nbytes, int32_t reps, double* avg_ms) {
+    if (nbytes <= 0 || reps <= 0) return HS_E_ARG;
+    hs_ctx tmp;
+    HS_CHECK(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+    double *a = nullptr, *b = nullptr;
+    const long n = (long)(nbytes / 8);
+    if (hipMalloc((void**)&a, n * 8) != hipSuccess || hipMalloc((void**)&b, n * 8) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        (void)hipStreamDestroy(tmp.stream);
+        return HS_E_NOMEM;
+    }
+    int e = launch_fill(&tmp, a, n, 1.0);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, tmp.stream);
+    for (int r = 0; r < reps && e == HS_OK; ++r) e = launch_copy(&tmp, b, a, n);
+    (void)hipEventRecord(e1, tmp.stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (avg_ms) *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipStreamDestroy(tmp.stream);
+    tmp.stream = nullptr;
+    return e;
 }
 
 }  // extern "C"

@@ -1,5 +1,5 @@
 // hs_device.h — device helpers shared by the kernel files: wave64 / block sums and the
-// deterministic "last block finishes" reduction.
+// deterministic two-kernel reduction (block partials + one-block finish).
 #pragma once
 
 #include "hs_internal.h"
@@ -54,60 +54,53 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, int tid) {
     b = rb;
 }
 
-// Writes the block partial (agent-scope store), bumps the counter with acq_rel; the
-// block that finishes last sums all partials in a fixed order.  Returns true in every
-// thread of that block; *out valid in tid 0.  The counter is reset for the next use.
+// Reductions: every block stores its partial sum with a plain store; a separate
+// one-block finish kernel, launched next on the same stream, adds the partials in a
+// fixed order and runs the scalar control step.  Stream order makes the partials
+// visible without per-block device-scope release/acquire (on a multi-XCD device each
+// such release writes back the XCD's L2), and the fixed order keeps results
+// deterministic run to run.
 template <int NT>
-__device__ bool reduce_last(double v, int tid, int bid, int nblocks, double* partials, unsigned int* counter,
-                            double* out) {
-    __shared__ int am_last;
+__device__ __forceinline__ void put_partial(double v, int tid, int bid, double* partials) {
     const double bs = block_sum<NT>(v, tid);
-    if (tid == 0) {
-        __hip_atomic_store(&partials[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double s = 0.0;
-    for (int i = tid; i < nblocks; i += NT) s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = block_sum<NT>(s, tid);
-    if (tid == 0) {
-        *out = s;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
+    if (tid == 0) partials[bid] = bs;
 }
 
-// Same for two sums (partials laid out [2][nblocks]).
 template <int NT>
-__device__ bool reduce2_last(double a, double b, int tid, int bid, int nblocks, double* partials,
-                             unsigned int* counter, double* outa, double* outb) {
-    __shared__ int am_last;
+__device__ __forceinline__ void put_partial2(double a, double b, int tid, int bid, int nblocks, double* partials) {
     block_sum2<NT>(a, b, tid);
     if (tid == 0) {
-        __hip_atomic_store(&partials[bid], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[nblocks + bid], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)(nblocks - 1));
+        partials[bid] = a;
+        partials[nblocks + bid] = b;
     }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double sa = 0.0, sb = 0.0;
-    for (int i = tid; i < nblocks; i += NT) {
-        sa += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sb += __hip_atomic_load(&partials[nblocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum2<NT>(sa, sb, tid);
-    if (tid == 0) {
-        *outa = sa;
-        *outb = sb;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
 }
+
+template <int NT>
+__device__ __forceinline__ double sum_partials(const double* partials, int n, int tid) {
+    double s = 0.0;
+    for (int i = tid; i < n; i += NT) s += partials[i];
+    return block_sum<NT>(s, tid);
+}
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs (each
+// with its own L2), so launch-order neighbours sit on different L2s.  A 1-D launch of
+// nb blocks is remapped so that XCD x owns one contiguous run of logical tiles (x fastest,
+// then y, then z): tiles that share halo rows / planes share an L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+    const int q = nb / NUM_XCD, rem = nb % NUM_XCD;
+    const int x = bid % NUM_XCD, r = bid / NUM_XCD;
+    return (x < rem) ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+struct Tile {
+    int bx, by, bz;
+};
+__device__ __forceinline__ Tile tile_of(int gx, int gy, int gz) {
+    const int nb = gx * gy * gz;
+    const int L = xcd_tile((int)blockIdx.x, nb);
+    return Tile{L % gx, (L / gx) % gy, L / (gx * gy)};
+}
+inline dim3 grid1(const dim3& g3) { return dim3(g3.x * g3.y * g3.z); }
 
 inline int kchunk(int nk, long tiles) {
     long want = 4096 / (tiles > 0 ? tiles : 1);

@@ -142,7 +142,6 @@ struct hs_ctx {
     double* dres = nullptr;
     // reductions
     double* partials = nullptr;
-    unsigned int* counter = nullptr;
     SolveCtl* sctl = nullptr;
     SweepCtl* wctl = nullptr;
     SolveCtl* h_sctl = nullptr;         // pinned mirrors
@@ -172,7 +171,21 @@ struct hs_ctx {
 int prof_begin(hs_ctx* c, int kernel);   // returns slot or -1
 int prof_end(hs_ctx* c, int slot);
 
+// kinds of the one-block reduction finish (k_finish)
+enum FinishKind : int {
+    FIN_SWEEP = 0,      // Gauss-Seidel loop control (wctl)
+    FIN_CG_SETUP,       // <b,b>
+    FIN_CG_ALPHA,       // <p,Ap> -> alpha, breakdown checks
+    FIN_CG_UPDATE,      // <r,r> -> HYPRE stop rule, beta
+    FIN_DOT,            // plain sum -> *out
+    FIN_FCG_SETUP,      // <b,b>
+    FIN_FCG_START,      // <r,z> -> gamma
+    FIN_FCG_UPDATE,     // <r,r> -> stop rule
+    FIN_FCG_BETA,       // <z,r>, <z,q> -> beta, gamma
+};
+
 // ---- kernel launchers (defined in the .hip files) ----
+int launch_finish(hs_ctx* c, int kind, int n, double* out);
 int launch_fill(hs_ctx* c, double* a, long n, double v);
 int launch_copy(hs_ctx* c, double* dst, const double* src, long n);
 int bc_x_ref(hs_ctx* c);

@@ -8,9 +8,9 @@
 //   (plane s-1) and the old i+1/j+1/k+1 neighbours (plane s+1).
 // Red-black ordering is the fast path: two fully parallel colour passes per sweep.
 //
-// Reductions are deterministic: per-block partial sums, then the last block to
-// finish (agent-scope acq_rel counter) adds the partials in a fixed order and runs
-// the scalar control step (CG alpha/beta/stop rule, Gauss-Seidel stop rule) on the
+// Reductions are deterministic: every block stores a partial sum, then a one-block
+// finish kernel (next on the stream) adds the partials in a fixed order and runs the
+// scalar control step (CG alpha/beta/stop rule, Gauss-Seidel stop rule) on the
 // device, so no host round trip is needed per iteration.
 #include <cfloat>
 #include <cmath>
@@ -27,7 +27,7 @@ using hsd::BT;
 using hsd::FLAT;
 using hsd::FLAT_BLOCKS;
 using hsd::block_sum;
-using hsd::reduce_last;
+using hsd::put_partial;
 using hsd::wave_sum;
 
 __device__ __forceinline__ double posd(double a) { return (a < 0.0) ? 0.0 : a; }
@@ -296,10 +296,10 @@ __device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long
 }
 
 // Red-black colour pass over the face box [1..ni] x [1..nj] x [1..nk]; per-block
-// partial of the squared change, control step in the last block after colour 1.
+// partial of the squared change; the finish kernel runs the control step after colour 1.
 template <int COMP>
 __global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour,
-                                               double* partials, unsigned* counter, SweepCtl* wctl) {
+                                               double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
     const int tid = threadIdx.x + BX * threadIdx.y;
     const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
@@ -314,31 +314,7 @@ __global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int nb = gridDim.x * gridDim.y * gridDim.z;
-    double* part = partials + (colour ? nb : 0);
-    double tot;
-    if (colour == 0) {
-        const double bs = block_sum<BT>(acc, tid);
-        if (tid == 0) part[bid] = bs;
-        return;
-    }
-    // colour 1: combine both colours' partials (fixed order) in the last block
-    const double bs = block_sum<BT>(acc, tid);
-    __shared__ int am_last;
-    if (tid == 0) {
-        __hip_atomic_store(&part[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    double s = 0.0;
-    for (int t = tid; t < 2 * nb; t += BT) s += __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    tot = block_sum<BT>(s, tid);
-    if (tid == 0) {
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sweep_control(wctl, tot);
-    }
+    put_partial<BT>(acc, tid, bid, partials + (colour ? nb : 0));
 }
 
 // One hyperplane i+j+k = s of the lexicographic sweep; dres gets the squared change.
@@ -359,15 +335,12 @@ __global__ __launch_bounds__(256) void k_mom_plane(MomArgs a, int ni, int nj, in
     (void)nlayout;
 }
 
-// Deterministic sum of dres[0..n) followed by the sweep control step.
-__global__ __launch_bounds__(FLAT) void k_sum_control(const double* x, long n, double* partials, unsigned* counter,
-                                                      SweepCtl* wctl) {
+// Partial sums of dres[0..n) (the sweep's squared changes); k_finish runs the control step.
+__global__ __launch_bounds__(FLAT) void k_sum_control(const double* x, long n, double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
     double acc = 0.0;
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) acc += x[t];
-    double tot;
-    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0)
-        sweep_control(wctl, tot);
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
 // ------------------------------------------------------------- energy ---
@@ -465,7 +438,7 @@ __device__ __forceinline__ bool t_update(const TArgs& a, long i, long j, long k,
 }
 
 __global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int kc, int colour,
-                                             double* partials, unsigned* counter, SweepCtl* wctl) {
+                                             double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
     const int tid = threadIdx.x + BX * threadIdx.y;
     const long i = i0 + blockIdx.x * BX + threadIdx.x, j = j0 + blockIdx.y * BY + threadIdx.y;
@@ -480,28 +453,7 @@ __global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, in
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int nb = gridDim.x * gridDim.y * gridDim.z;
-    double* part = partials + (colour ? nb : 0);
-    const double bs = block_sum<BT>(acc, tid);
-    if (colour == 0) {
-        if (tid == 0) part[bid] = bs;
-        return;
-    }
-    __shared__ int am_last;
-    if (tid == 0) {
-        __hip_atomic_store(&part[bid], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    double s = 0.0;
-    for (int t = tid; t < 2 * nb; t += BT) s += __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double tot = block_sum<BT>(s, tid);
-    if (tid == 0) {
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sweep_control(wctl, tot);
-    }
+    put_partial<BT>(acc, tid, bid, partials + (colour ? nb : 0));
 }
 
 __global__ __launch_bounds__(256) void k_t_plane(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int s,
@@ -592,62 +544,72 @@ struct OpArgs {
 };
 
 // y = A x at one fluid cell: diagonal first, then B, S, W, E, N, T (CSR column order
-// of fluid_solver.h:2003-2083).  Entries of x outside fluid cells are zero.
-__device__ __forceinline__ double apply_row(const OpArgs& o, const double* x, long i, long j, long k, long c) {
+// of fluid_solver.h:2003-2083).  Entries of x outside fluid cells are zero.  xb/xc/xt
+// are x at k-1, k, k+1 and cb/ct the bottom/top z-face coefficients, carried in
+// registers by the z-marching callers.
+__device__ __forceinline__ double apply_row(const OpArgs& o, const double* x, long c, long cu, long cv,
+                                            double xb, double xc, double xt, double cB, double cT) {
     const Dims& d = o.d;
-    double cB, cS, cW, cE, cN, cT, dg;
+    double cS, cW, cE, cN, dg;
     if (o.modeR) {
-        cW = cE = o.ax[c]; cS = cN = o.ay[c]; cB = cT = o.az[c];
+        cW = cE = o.ax[c]; cS = cN = o.ay[c];
         dg = cE + cW + cN + cS + cT + cB;
     } else {
-        cW = o.ax[idxU(d, i, j, k)]; cE = o.ax[idxU(d, i + 1, j, k)];
-        cS = o.ay[idxV(d, i, j, k)]; cN = o.ay[idxV(d, i, j + 1, k)];
-        cB = o.az[idxW(d, i, j, k)]; cT = o.az[idxW(d, i, j, k + 1)];
+        cW = o.ax[cu]; cE = o.ax[cu + 1];
+        cS = o.ay[cv]; cN = o.ay[cv + d.sVy];
         dg = cW + cE + cS + cN + cB + cT;
     }
-    double t = dg * x[c];
-    t += (-cB) * x[c - d.sPz];
+    double t = dg * xc;
+    t += (-cB) * xb;
     t += (-cS) * x[c - d.sPy];
     t += (-cW) * x[c - 1];
     t += (-cE) * x[c + 1];
     t += (-cN) * x[c + d.sPy];
-    t += (-cT) * x[c + d.sPz];
+    t += (-cT) * xt;
     return t;
 }
 
-// s = A p (z-marching columns) with the partial of <s,p>; last block: alpha.
-__global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, double* s, int kc, double* partials,
-                                                unsigned* counter, SolveCtl* ctl) {
-    if (ctl->done) return;
+// Walks one (i,j) column over planes k0..k1 calling f(c, y) for fluid cells.
+template <typename F>
+__device__ __forceinline__ void spmv_column(const OpArgs& o, const double* x, long i, long j, int k0, int k1, F f) {
     const Dims& d = o.d;
-    const int tid = threadIdx.x + BX * threadIdx.y;
-    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
-    double acc = 0.0;
-    if (i <= d.nx && j <= d.ny) {
-        const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
-        for (int k = k0; k <= k1; ++k) {
-            const long c = idxP(d, i, j, k);
-            if (o.cls[c] != CELL_FLUID) continue;
-            const double y = apply_row(o, p, i, j, k, c);
-            s[c] = y;
-            acc += y * p[c];
-        }
-    }
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    double tot;
-    if (reduce_last<BT>(acc, tid, bid, nb, partials, counter, &tot) && tid == 0) {
-        ctl->sdotp = tot;
-        if (tot == 0.0) { ctl->done = 1; return; }
-        const double alpha = ctl->gamma / tot;
-        if (!(alpha > DBL_MIN)) { ctl->done = 1; return; }
-        ctl->alpha = alpha;
+    long c = idxP(d, i, j, k0), cu = idxU(d, i, j, k0), cv = idxV(d, i, j, k0);
+    double xb = x[c - d.sPz], xc = x[c];
+    // z-face coefficients: Mode R row coefficient az at the cell (both faces), Mode C the
+    // W-layout faces k and k+1 (W shares the P strides)
+    double cb = o.az[c];
+    for (int k = k0; k <= k1; ++k, c += d.sPz, cu += d.sUz, cv += d.sVz) {
+        const double xt = x[c + d.sPz];
+        const double ct = o.modeR ? o.az[c] : o.az[c + d.sPz];
+        if (o.cls[c] == CELL_FLUID) f(c, apply_row(o, x, c, cu, cv, xb, xc, xt, o.modeR ? ct : cb, ct));
+        xb = xc;
+        xc = xt;
+        cb = o.modeR ? o.az[c + d.sPz] : ct;
     }
 }
 
-// x += alpha p, r -= alpha s, partial <r,r>; last block: HYPRE stop rule and beta.
+// s = A p (z-marching columns) with block partials of <s,p> (k_finish: alpha).
+__global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, double* s, int kc, int gx, int gy, int gz,
+                                                double* partials, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const Dims& d = o.d;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
+        spmv_column(o, p, i, j, k0, k1, [&](long c, double y) {
+            s[c] = y;
+            acc += y * p[c];
+        });
+    }
+    put_partial<BT>(acc, tid, blockIdx.x, partials);
+}
+
+// x += alpha p, r -= alpha s, block partials of <r,r> (k_finish: stop rule, beta)
 __global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* x, double* r, const double* p, const double* s,
-                                                    double* partials, unsigned* counter, SolveCtl* ctl) {
+                                                    double* partials, SolveCtl* ctl) {
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double acc = 0.0;
@@ -657,16 +619,7 @@ __global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* x, double* r
         r[t] = rv;
         acc += rv * rv;
     }
-    double tot;
-    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
-        ctl->iters += 1;
-        ctl->gamma_old = ctl->gamma;
-        ctl->gamma = tot;
-        ctl->rr = tot;
-        if (tot / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; return; }
-        if (ctl->iters >= ctl->maxit) { ctl->done = 1; return; }
-        ctl->beta = ctl->gamma / ctl->gamma_old;
-    }
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
 // p = r + beta p
@@ -676,46 +629,89 @@ __global__ __launch_bounds__(FLAT) void k_cg_pupdate(long n, const double* r, do
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) p[t] = r[t] + beta * p[t];
 }
 
-// generic dot into ctl->red[slot]
-__global__ __launch_bounds__(FLAT) void k_dot(long n, const double* a, const double* b, double* partials, unsigned* counter,
-                                              double* out) {
+// block partials of <a,b>
+__global__ __launch_bounds__(FLAT) void k_dot(long n, const double* a, const double* b, double* partials) {
     double acc = 0.0;
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) acc += a[t] * b[t];
-    double tot;
-    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) *out = tot;
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// CG setup: r = p = b, x = 0 (x0 = 0, fluid_solver.h:1766); <b,b> -> ctl
+// CG setup: r = p = b, x = 0 (x0 = 0, fluid_solver.h:1766); partials of <b,b>
 __global__ __launch_bounds__(FLAT) void k_cg_setup(long n, const double* b, double* x, double* r, double* p,
-                                                   double* partials, unsigned* counter, SolveCtl* ctl) {
+                                                   double* partials) {
     double acc = 0.0;
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
         const double bv = b[t];
         x[t] = 0.0; r[t] = bv; p[t] = bv;
         acc += bv * bv;
     }
-    double tot;
-    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
-        ctl->bb = tot;
-        ctl->gamma = tot;
-        ctl->rr = tot;
-        ctl->iters = 0;
-        ctl->converged = 0;
-        ctl->done = (tot == 0.0) ? 1 : 0;
-        if (tot == 0.0) ctl->converged = 1;
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
+}
+
+// One-block finish of a reduction: adds the block partials in a fixed order and runs
+// the scalar step of the kernel that produced them.
+
+__global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, int kind, SolveCtl* ctl, SweepCtl* wctl,
+                                                 double* out) {
+    if (kind == FIN_SWEEP) {
+        if (!wctl->active) return;
+    } else if (kind == FIN_CG_ALPHA || kind == FIN_CG_UPDATE || kind == FIN_FCG_START || kind == FIN_FCG_UPDATE ||
+               kind == FIN_FCG_BETA) {
+        if (ctl->done) return;
+    }
+    const int tid = threadIdx.x;
+    const double a = hsd::sum_partials<1024>(partials, n, tid);
+    double b = 0.0;
+    if (kind == FIN_FCG_BETA) b = hsd::sum_partials<1024>(partials + n, n, tid);
+    if (tid != 0) return;
+    switch (kind) {
+        case FIN_SWEEP: sweep_control(wctl, a); break;
+        case FIN_CG_SETUP:
+            ctl->bb = a; ctl->gamma = a; ctl->rr = a; ctl->iters = 0;
+            ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
+            break;
+        case FIN_CG_ALPHA: {
+            ctl->sdotp = a;
+            if (a == 0.0) { ctl->done = 1; break; }
+            const double alpha = ctl->gamma / a;
+            if (!(alpha > DBL_MIN)) { ctl->done = 1; break; }
+            ctl->alpha = alpha;
+            break;
+        }
+        case FIN_CG_UPDATE:
+            ctl->iters += 1; ctl->gamma_old = ctl->gamma; ctl->gamma = a; ctl->rr = a;
+            if (a / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; break; }
+            if (ctl->iters >= ctl->maxit) { ctl->done = 1; break; }
+            ctl->beta = ctl->gamma / ctl->gamma_old;
+            break;
+        case FIN_DOT: *out = a; break;
+        case FIN_FCG_SETUP:
+            ctl->bb = a; ctl->rr = a; ctl->iters = 0;
+            ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
+            break;
+        case FIN_FCG_START: ctl->gamma = a; break;
+        case FIN_FCG_UPDATE:
+            ctl->iters += 1; ctl->rr = a;
+            if (a / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; break; }
+            if (ctl->iters >= ctl->maxit) { ctl->done = 1; break; }
+            break;
+        case FIN_FCG_BETA:
+            ctl->beta = -ctl->alpha * b / ctl->gamma;
+            ctl->gamma_old = ctl->gamma;
+            ctl->gamma = a;
+            break;
     }
 }
 
-// plain y = A x (fluid cells) without reduction
-__global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* y, int kc) {
+// plain y = A x (fluid cells; zero elsewhere) without reduction
+__global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* y, int kc, int gx, int gy, int gz) {
     const Dims& d = o.d;
-    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
     if (i > d.nx || j > d.ny) return;
-    const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
-    for (int k = k0; k <= k1; ++k) {
-        const long c = idxP(d, i, j, k);
-        y[c] = (o.cls[c] == CELL_FLUID) ? apply_row(o, x, i, j, k, c) : 0.0;
-    }
+    const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
+    for (int k = k0; k <= k1; ++k) y[idxP(d, i, j, k)] = 0.0;
+    spmv_column(o, x, i, j, k0, k1, [&](long c, double v) { y[c] = v; });
 }
 
 // zero entries outside fluid cells
@@ -723,7 +719,7 @@ __global__ void k_mask_fluid(long n, const uint8_t* cls, double* x) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
         if (cls[t] != CELL_FLUID) x[t] = 0.0;
 }
-// r = b - A x entries for the residual check
+// z = a x + b y (residual check)
 __global__ void k_axpby(long n, double a, const double* x, double b, const double* y, double* z) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) z[t] = a * x[t] + b * y[t];
 }
@@ -780,7 +776,7 @@ __global__ void k_update_p(Dims d, const uint8_t* cls, const double* pp, double*
 }
 
 // Mode C energy balance: sums of cell residuals and of heat input.
-__global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* partials, unsigned* counter, double* out) {
+__global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* partials) {
     const Dims& d = a.d;
     const int tid = threadIdx.x + BX * threadIdx.y;
     const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
@@ -798,20 +794,17 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
         }
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    double tot;
-    if (reduce_last<BT>(acc, tid, bid, nb, partials, counter, &tot) && tid == 0) *out = tot;
+    put_partial<BT>(acc, tid, bid, partials);
 }
 
-__global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, double* partials, unsigned* counter, double* out) {
+__global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, double* partials) {
     double acc = 0.0;
     const long n = d.nC;
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
         const long i = t % d.nx, j = (t / d.nx) % d.ny, k = t / ((long)d.nx * d.ny);
         acc += q[t] * (g.xp[i + 1] * g.yp[j + 1] * g.zp[k + 1]);
     }
-    double tot;
-    if (reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) *out = tot;
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
 }  // namespace
@@ -820,6 +813,7 @@ __global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, doub
 namespace {
 using hsd::kchunk;
 using hsd::grid3;
+using hsd::grid1;
 MomArgs mom_args(hs_ctx* c, double* ap) {
     MomArgs a;
     a.d = c->d; a.g = c->g; a.u = c->u; a.v = c->v; a.w = c->w; a.ap = ap;
@@ -844,13 +838,18 @@ OpArgs op_args(hs_ctx* c) {
 }
 }  // namespace
 
+int launch_finish(hs_ctx* c, int kind, int n, double* out) {
+    k_finish<<<1, 1024, 0, c->stream>>>(c->partials, n, kind, c->sctl, c->wctl, out);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
 int launch_fill(hs_ctx* c, double* a, long n, double v) {
-    k_fill<<<1024, 256, 0, c->stream>>>(a, n, v);
+    k_fill<<<FLAT_BLOCKS, 256, 0, c->stream>>>(a, n, v);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 int launch_copy(hs_ctx* c, double* dst, const double* src, long n) {
-    k_copy<<<1024, 256, 0, c->stream>>>(dst, src, n);
+    k_copy<<<FLAT_BLOCKS, 256, 0, c->stream>>>(dst, src, n);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
@@ -939,12 +938,13 @@ int momentum_sweep(hs_ctx* c, int comp) {
         const dim3 gr = grid3(ni, nj, nk, kc);
         for (int colour = 0; colour < 2; ++colour) {
             const int slot = comp == 0 ? prof_begin(c, HS_K_MOM_RB) : -1;
-            if (comp == 0) k_mom_rb<0><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
-            else if (comp == 1) k_mom_rb<1><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
-            else k_mom_rb<2><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            if (comp == 0) k_mom_rb<0><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
+            else if (comp == 1) k_mom_rb<1><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
+            else k_mom_rb<2><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
             HS_TRY(prof_end(c, slot));
         }
+        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * (int)(gr.x * gr.y * gr.z), nullptr));
     } else {
         const long nl = comp == 0 ? d.nU : (comp == 1 ? d.nV : d.nW);
         HS_TRY(launch_fill(c, c->dres, nl, 0.0));
@@ -956,8 +956,9 @@ int momentum_sweep(hs_ctx* c, int comp) {
             else k_mom_plane<2><<<nb, 256, 0, c->stream>>>(a, ni, nj, nk, s, c->wctl, nl);
         }
         HS_CHECK(hipGetLastError());
-        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, nl, c->partials, c->counter, c->wctl);
+        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, nl, c->partials, c->wctl);
         HS_CHECK(hipGetLastError());
+        HS_TRY(launch_finish(c, FIN_SWEEP, FLAT_BLOCKS, nullptr));
     }
     return HS_OK;
 }
@@ -970,9 +971,10 @@ static int t_sweep_box(hs_ctx* c, bool solid, int i0, int j0, int k0, int ni, in
         const int kc = kchunk(nk, tiles);
         const dim3 gr = grid3(ni, nj, nk, kc);
         for (int colour = 0; colour < 2; ++colour) {
-            k_t_rb<<<gr, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, c->partials, c->counter, c->wctl);
+            k_t_rb<<<gr, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
         }
+        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * (int)(gr.x * gr.y * gr.z), nullptr));
     } else {
         HS_TRY(launch_fill(c, c->dres, d.nP, 0.0));
         const long njk = (long)nj * nk;
@@ -980,8 +982,9 @@ static int t_sweep_box(hs_ctx* c, bool solid, int i0, int j0, int k0, int ni, in
         for (int s = 0; s <= (ni - 1) + (nj - 1) + (nk - 1); ++s)
             k_t_plane<<<nb, 256, 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, s, c->wctl);
         HS_CHECK(hipGetLastError());
-        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, d.nP, c->partials, c->counter, c->wctl);
+        k_sum_control<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->dres, d.nP, c->partials, c->wctl);
         HS_CHECK(hipGetLastError());
+        HS_TRY(launch_finish(c, FIN_SWEEP, FLAT_BLOCKS, nullptr));
     }
     return HS_OK;
 }
@@ -1026,14 +1029,16 @@ int spmv(hs_ctx* c, const double* x, double* y) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
-    k_spmv<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(op_args(c), x, y, kc);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    k_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), x, y, kc, gr.x, gr.y, gr.z);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 
 int dot(hs_ctx* c, const double* a, const double* b, double* out_host) {
-    k_dot<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->d.nP, a, b, c->partials, c->counter, &c->sctl->red[0]);
+    k_dot<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(c->d.nP, a, b, c->partials);
     HS_CHECK(hipGetLastError());
+    HS_TRY(launch_finish(c, FIN_DOT, FLAT_BLOCKS, &c->sctl->red[0]));
     HS_CHECK(hipMemcpyAsync(out_host, &c->sctl->red[0], sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
     return HS_OK;
@@ -1050,8 +1055,9 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     h->maxit = c->prm.p_maxit;
     HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    k_cg_setup<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, b, x, c->kr, c->kp, c->partials, c->counter, c->sctl);
+    k_cg_setup<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, b, x, c->kr, c->kp, c->partials);
     HS_CHECK(hipGetLastError());
+    HS_TRY(launch_finish(c, FIN_CG_SETUP, FLAT_BLOCKS, nullptr));
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
     const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
@@ -1061,9 +1067,11 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
             const int slot = prof_begin(c, HS_K_SPMV);
-            k_cg_spmv<<<gr, dim3(BX, BY), 0, c->stream>>>(o, c->kp, c->ks, kc, c->partials, c->counter, c->sctl);
+            k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, c->kp, c->ks, kc, gr.x, gr.y, gr.z, c->partials, c->sctl);
             HS_TRY(prof_end(c, slot));
-            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, c->kp, c->ks, c->partials, c->counter, c->sctl);
+            HS_TRY(launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr));
+            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, c->kp, c->ks, c->partials, c->sctl);
+            HS_TRY(launch_finish(c, FIN_CG_UPDATE, FLAT_BLOCKS, nullptr));
             k_cg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, c->kr, c->kp, c->sctl);
         }
         HS_CHECK(hipGetLastError());
@@ -1084,7 +1092,7 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
 
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm) {
     HS_TRY(spmv(c, x, c->kq));
-    k_axpby<<<1024, 256, 0, c->stream>>>(c->d.nP, 1.0, b, -1.0, c->kq, c->kq);
+    k_axpby<<<FLAT_BLOCKS, 256, 0, c->stream>>>(c->d.nP, 1.0, b, -1.0, c->kq, c->kq);
     HS_CHECK(hipGetLastError());
     double rr, bb;
     HS_TRY(dot(c, c->kq, c->kq, &rr));
@@ -1114,11 +1122,13 @@ int energy_balance(hs_ctx* c, double* heat, double* resid) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
-    k_energy_balance<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials,
-                                                                                 c->counter, &c->sctl->red[1]);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials);
     HS_CHECK(hipGetLastError());
-    k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials, c->counter, &c->sctl->red[2]);
+    HS_TRY(launch_finish(c, FIN_DOT, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
+    k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials);
     HS_CHECK(hipGetLastError());
+    HS_TRY(launch_finish(c, FIN_DOT, FLAT_BLOCKS, &c->sctl->red[2]));
     double h2[2];
     HS_CHECK(hipMemcpyAsync(h2, &c->sctl->red[1], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
@@ -1138,9 +1148,10 @@ int cg_spmv_dot(hs_ctx* c, const double* p, double* q) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
     const int slot = prof_begin(c, HS_K_SPMV);
-    k_cg_spmv<<<grid3(d.nx, d.ny, d.nz, kc), dim3(BX, BY), 0, c->stream>>>(op_args(c), p, q, kc, c->partials,
-                                                                           c->counter, c->sctl);
+    k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), p, q, kc, gr.x, gr.y, gr.z, c->partials, c->sctl);
     HS_CHECK(hipGetLastError());
-    return prof_end(c, slot);
+    HS_TRY(prof_end(c, slot));
+    return launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr);
 }

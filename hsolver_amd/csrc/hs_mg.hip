@@ -137,11 +137,13 @@ __device__ __forceinline__ T offsum(const Lv<T>& L, long c) {
 
 // one colour of red-black Gauss-Seidel
 template <typename T>
-__global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, const SolveCtl* ctl) {
+__global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, int gx, int gy, int gz,
+                                                  const SolveCtl* ctl) {
     if (ctl->done) return;
-    const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
     if (i > L.nx || j > L.ny) return;
-    const int k0 = 1 + blockIdx.z * kc, k1 = min(L.nz, k0 + kc - 1);
+    const int k0 = 1 + tl.bz * kc, k1 = min(L.nz, k0 + kc - 1);
     for (int k = k0; k <= k1; ++k) {
         if (((i + j + k) & 1) != colour) continue;
         const long c = i + L.sy * j + L.sz * k;
@@ -281,99 +283,78 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(LvPack<T> P, int t, in
     }
 }
 
+// ---- flexible CG pieces (FP64 outer vectors; the level-0 V-cycle arrays are T) ----
+// Each reduction kernel stores block partials; k_finish (hs_kernels.hip) adds them.
+#define GRID_STRIDE(t, n) for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < (n); t += (long)gridDim.x * FLAT)
+
+// x = 0, r = b, level-0 right-hand side = (T) b, level-0 iterate = 0; partials of <b,b>
 template <typename T>
-__global__ void k_mg_load(long n, const double* r, T* b, T* x, const SolveCtl* ctl) {
-    if (ctl->done) return;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        b[t] = (T)r[t];
-        x[t] = T(0);
+__global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, double* x, double* r, T* b0, T* x0,
+                                                    double* partials) {
+    double acc = 0.0;
+    GRID_STRIDE(t, n) {
+        const double bv = b[t];
+        x[t] = 0.0;
+        r[t] = bv;
+        b0[t] = (T)bv;
+        x0[t] = T(0);
+        acc += bv * bv;
     }
+    hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
+// p = z (the level-0 iterate); partials of <r, z>
 template <typename T>
-__global__ void k_mg_store(long n, const T* x, double* z, const SolveCtl* ctl) {
-    if (ctl->done) return;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) z[t] = (double)x[t];
-}
-
-// ---- flexible CG pieces (FP64) ----
-// p = z, gamma = <r, z>
-__global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const double* z, double* p,
-                                                    double* partials, unsigned* counter, SolveCtl* ctl) {
+__global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const T* z, double* p, double* partials,
+                                                    const SolveCtl* ctl) {
     if (ctl->done) return;
     double acc = 0.0;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
-        p[t] = z[t];
-        acc += r[t] * z[t];
+    GRID_STRIDE(t, n) {
+        const double zv = (double)z[t];
+        p[t] = zv;
+        acc += r[t] * zv;
     }
-    double tot;
-    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0)
-        ctl->gamma = tot;
+    hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// x += alpha p, r -= alpha q, <r,r>; stop rule
+// x += alpha p, r -= alpha q; level-0 right-hand side = (T) r, iterate = 0; partials of <r,r>
+template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const double* p, const double* q,
-                                                     double* partials, unsigned* counter, SolveCtl* ctl) {
+                                                     T* b0, T* x0, double* partials, const SolveCtl* ctl) {
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double acc = 0.0;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
+    GRID_STRIDE(t, n) {
         x[t] += alpha * p[t];
         const double rv = r[t] - alpha * q[t];
         r[t] = rv;
         acc += rv * rv;
+        b0[t] = (T)rv;
+        x0[t] = T(0);
     }
-    double tot;
-    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
-        ctl->iters += 1;
-        ctl->rr = tot;
-        if (tot / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; return; }
-        if (ctl->iters >= ctl->maxit) { ctl->done = 1; return; }
-    }
+    hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// gamma_new = <z, r>, beta = -alpha <z, q> / gamma_old
-__global__ __launch_bounds__(FLAT) void k_fcg_beta(long n, const double* z, const double* r, const double* q,
-                                                   double* partials, unsigned* counter, SolveCtl* ctl) {
+// partials of <z, r> and <z, q> (z = level-0 iterate)
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_fcg_beta(long n, const T* z, const double* r, const double* q,
+                                                   double* partials, const SolveCtl* ctl) {
     if (ctl->done) return;
     double a = 0.0, b = 0.0;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
-        a += z[t] * r[t];
-        b += z[t] * q[t];
+    GRID_STRIDE(t, n) {
+        const double zv = (double)z[t];
+        a += zv * r[t];
+        b += zv * q[t];
     }
-    double zr, zq;
-    if (hsd::reduce2_last<FLAT>(a, b, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &zr, &zq) &&
-        threadIdx.x == 0) {
-        ctl->beta = -ctl->alpha * zq / ctl->gamma;
-        ctl->gamma_old = ctl->gamma;
-        ctl->gamma = zr;
-    }
+    hsd::put_partial2<FLAT>(a, b, threadIdx.x, blockIdx.x, gridDim.x, partials);
 }
 
 // p = z + beta p
-__global__ __launch_bounds__(FLAT) void k_fcg_pupdate(long n, const double* z, double* p, const SolveCtl* ctl) {
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_fcg_pupdate(long n, const T* z, double* p, const SolveCtl* ctl) {
     if (ctl->done) return;
     const double beta = ctl->beta;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) p[t] = z[t] + beta * p[t];
-}
-
-__global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, double* x, double* r, double* partials,
-                                                    unsigned* counter, SolveCtl* ctl) {
-    double acc = 0.0;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
-        const double bv = b[t];
-        x[t] = 0.0;
-        r[t] = bv;
-        acc += bv * bv;
-    }
-    double tot;
-    if (hsd::reduce_last<FLAT>(acc, threadIdx.x, blockIdx.x, gridDim.x, partials, counter, &tot) && threadIdx.x == 0) {
-        ctl->bb = tot;
-        ctl->rr = tot;
-        ctl->iters = 0;
-        ctl->converged = (tot == 0.0) ? 1 : 0;
-        ctl->done = (tot == 0.0) ? 1 : 0;
-    }
+    GRID_STRIDE(t, n) p[t] = (double)z[t] + beta * p[t];
 }
 
 // ------------------------------------------------------------ host side ---
@@ -397,19 +378,18 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
             const int slot = m->level == 0 ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
-            k_mg_smooth<T><<<gr, dim3(BX, BY), 0, c->stream>>>(L, kc, colour, c->sctl);
+            k_mg_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, kc, colour, gr.x, gr.y, gr.z, c->sctl);
             HS_TRY(prof_end(c, slot));
         }
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 
+// One V-cycle on the level-0 arrays: b holds the residual, x starts at 0 and leaves
+// holding the preconditioned residual (both prepared / consumed by the FCG kernels).
 template <typename T>
-int vcycle(hs_ctx* c, const double* r, double* z) {
+int vcycle(hs_ctx* c) {
     const int nl = (int)c->mg.size();
-    MGLevel* m0 = c->mg[0];
-    k_mg_load<T><<<1024, 256, 0, c->stream>>>(m0->n, r, (T*)m0->b, (T*)m0->x, c->sctl);
-    HS_CHECK(hipGetLastError());
     // first level small enough for the single-workgroup tail
     int t = 0;
     while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
@@ -431,8 +411,6 @@ int vcycle(hs_ctx* c, const double* r, double* z) {
         HS_CHECK(hipGetLastError());
         HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
     }
-    k_mg_store<T><<<1024, 256, 0, c->stream>>>(m0->n, (const T*)m0->x, z, c->sctl);
-    HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 
@@ -480,20 +458,27 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     h->maxit = c->prm.p_maxit;
     HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    double *r = c->kr, *p = c->kp, *q = c->ks, *z = c->kz;
-    k_fcg_setup<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, b, x, r, c->partials, c->counter, c->sctl);
-    HS_TRY(vcycle<T>(c, r, z));
-    k_fcg_start<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, r, z, p, c->partials, c->counter, c->sctl);
+    double *r = c->kr, *p = c->kp, *q = c->ks;
+    T* b0 = (T*)c->mg[0]->b;
+    T* z = (T*)c->mg[0]->x;
+    const int G = FLAT_BLOCKS;
+    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z, c->partials);
+    HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));
+    HS_TRY(vcycle<T>(c));
+    k_fcg_start<T><<<G, FLAT, 0, c->stream>>>(n, r, z, p, c->partials, c->sctl);
+    HS_TRY(launch_finish(c, FIN_FCG_START, G, nullptr));
     HS_CHECK(hipGetLastError());
     int launched = 0;
     int batch = c->last_iters > 2 ? c->last_iters - 1 : 4;
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
             HS_TRY(cg_spmv_dot(c, p, q));
-            k_fcg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, r, p, q, c->partials, c->counter, c->sctl);
-            HS_TRY(vcycle<T>(c, r, z));
-            k_fcg_beta<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, z, r, q, c->partials, c->counter, c->sctl);
-            k_fcg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, z, p, c->sctl);
+            k_fcg_update<T><<<G, FLAT, 0, c->stream>>>(n, x, r, p, q, b0, z, c->partials, c->sctl);
+            HS_TRY(launch_finish(c, FIN_FCG_UPDATE, G, nullptr));
+            HS_TRY(vcycle<T>(c));
+            k_fcg_beta<T><<<G, FLAT, 0, c->stream>>>(n, z, r, q, c->partials, c->sctl);
+            HS_TRY(launch_finish(c, FIN_FCG_BETA, G, nullptr));
+            k_fcg_pupdate<T><<<G, FLAT, 0, c->stream>>>(n, z, p, c->sctl);
         }
         HS_CHECK(hipGetLastError());
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));

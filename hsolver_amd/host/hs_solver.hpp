@@ -103,7 +103,8 @@ struct Solver {
 
 // Output gather + VTK writer of main.cpp:101-156 on one rank: one point per cell at
 // the cell's minimum vertex (solver.h:178-180), velocity sampled on the cell's
-// west/south/bottom faces (solver.h:172-174), with DIMENSIONS set from the grid.
+// west/south/bottom faces (solver.h:172-174), with DIMENSIONS set from the grid and
+// data printed at round-trip precision (%.17g).
 inline void write_vtk(const char* path, hs_ctx* c, const hs_grid& g) {
     const long nx = g.nx, ny = g.ny, nz = g.nz, n = nx * ny * nz;
     std::vector<double> u((nx + 1) * ny * nz), v(nx * (ny + 1) * nz), w(nx * ny * (nz + 1)), p(n), t(n);
@@ -116,15 +117,15 @@ inline void write_vtk(const char* path, hs_ctx* c, const hs_grid& g) {
         for (long j = 0; j < ny; ++j)
             for (long i = 0; i < nx; ++i) std::fprintf(f, "%.12f %.12f %.12f\n", g.xv[i], g.yv[j], g.zv[k]);
     std::fprintf(f, "POINT_DATA %ld\nSCALARS pressure double 1\nLOOKUP_TABLE default\n", n);
-    for (long q = 0; q < n; ++q) std::fprintf(f, "%.12f\n", p[q]);
+    for (long q = 0; q < n; ++q) std::fprintf(f, "%.17g\n", p[q]);
     std::fprintf(f, "VECTORS velocity double\n");
     for (long k = 0; k < nz; ++k)
         for (long j = 0; j < ny; ++j)
             for (long i = 0; i < nx; ++i)
-                std::fprintf(f, "%.12f %.12f %.12f\n", u[i + (nx + 1) * (j + ny * k)], v[i + nx * (j + (ny + 1) * k)],
+                std::fprintf(f, "%.17g %.17g %.17g\n", u[i + (nx + 1) * (j + ny * k)], v[i + nx * (j + (ny + 1) * k)],
                              w[i + nx * (j + ny * k)]);
     std::fprintf(f, "SCALARS temperature double 1\nLOOKUP_TABLE default\n");
-    for (long q = 0; q < n; ++q) std::fprintf(f, "%.12f\n", t[q]);
+    for (long q = 0; q < n; ++q) std::fprintf(f, "%.17g\n", t[q]);
     std::fprintf(f, "SCALARS id int 1\nLOOKUP_TABLE default\n");
     for (long q = 0; q < n; ++q) std::fprintf(f, "%ld\n", g.cell_type[q] == 0 ? q : -1L);
     std::fclose(f);

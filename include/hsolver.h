@@ -211,6 +211,9 @@ int hs_set_conductivity(hs_ctx* c, const double* k);
 int hs_prof_set(hs_ctx* c, int32_t kernel);
 /* launches timed and their summed duration in ms since hs_prof_set */
 int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms);
+/* plain device copy of nbytes (reps times): the known-size reference for hardware byte
+ * counters; *avg_ms = mean copy time */
+int hs_copy_bench(int64_t nbytes, int32_t reps, double* avg_ms);
 /* multigrid level dimensions (3 ints per level); returns the number of levels */
 int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels);
 
