@@ -16,6 +16,7 @@ HS_MODE_REF, HS_MODE_CONFIG = 0, 1
 HS_ORDER_LEX, HS_ORDER_RB = 0, 1
 HS_PRECOND_NONE, HS_PRECOND_GMG = 0, 1
 HS_PREC_FP64, HS_PREC_MIXED = 0, 1
+HS_MEM_HOST, HS_MEM_DEVICE = 0, 1
 
 (HS_A_U, HS_A_V, HS_A_W, HS_A_UP, HS_A_VP, HS_A_WP, HS_A_P, HS_A_PTEMP, HS_A_PPRIME,
  HS_A_TF, HS_A_TS, HS_A_Q, HS_A_K, HS_A_B) = range(14)
@@ -63,6 +64,15 @@ class Stats(C.Structure):
             "heat_in": self.heat_in, "mom_residual": list(self.mom_residual),
             "t_residual": self.t_residual, "ts_residual": self.ts_residual,
         }
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("breakdown", C.c_int32),
+                ("restarts", C.c_int32), ("relres", C.c_double), ("true_relres", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"iters": self.iters, "converged": self.converged, "breakdown": self.breakdown,
+                "restarts": self.restarts, "relres": self.relres, "true_relres": self.true_relres}
 
 
 def params_default(mode: int) -> Params:
@@ -170,7 +180,11 @@ LIB_PATH = PKG_DIR / "libhsolver_amd.so"
 
 
 def load_library(path: os.PathLike | None = None) -> C.CDLL:
-    """Load the HIP library.  Raises if it has not been built: there is no CPU path."""
+    """Load the HIP library.  Raises if it has not been built: there is no CPU path.
+
+    A process that also uses torch.cuda must import torch first: torch ships its own
+    HIP runtime, and the library binds to an already loaded one (loading the library
+    first maps /opt/rocm's runtime beside torch's, and torch then finds no device)."""
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise RuntimeError(f"{p} is missing: run __graft_entry__.build() (the HIP library is required)")
@@ -183,6 +197,8 @@ def declare(lib: C.CDLL, prefix: str) -> None:
     """Attach argtypes for the hs_/or_ entry points that exist in lib."""
     P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
     PS = C.POINTER(Stats)
+    PI = C.POINTER(SolveInfo)
+    D_ = C.c_double
     sig = {
         "create": (C.c_int, [C.POINTER(Grid), C.POINTER(Params), C.POINTER(P)]),
         "destroy": (None, [P]),
@@ -207,6 +223,12 @@ def declare(lib: C.CDLL, prefix: str) -> None:
         "set_source": (C.c_int, [P, D]),
         "set_conductivity": (C.c_int, [P, D]),
         "cell_bc": (C.c_int, [P]),
+        "energy_solve": (C.c_int, [P, D_, I32, PI]),
+        "adjoint_energy": (C.c_int, [P, P, P, P, D_, I32, I32, PI]),
+        "energy_apply": (C.c_int, [P, I32, P, P, I32]),
+        "energy_solve_rhs": (C.c_int, [P, I32, P, P, D_, I32, I32, PI]),
+        "get_cells": (C.c_int, [P, I32, P, I32]),
+        "put_cells": (C.c_int, [P, I32, P, I32]),
         "strerror": (C.c_char_p, [C.c_int]),
         "params_default": (None, [I32, C.POINTER(Params)]),
     }

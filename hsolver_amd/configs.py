@@ -158,10 +158,15 @@ def c3(n: int = 256) -> Case:
 
 
 def c4(n: int = 128) -> Case:
-    """c2 geometry (a centred cube of 1/4 the channel height) in a unit cube."""
+    """c2 geometry (a centred cube of 1/4 the channel height) in a unit cube.
+
+    Temperatures are rises above the inlet / wall value (t_wall = 0): the energy
+    equation is linear, so this is the same problem as any other wall temperature,
+    and the p-norm objective then measures the heating rather than the offset."""
     lo, hi = int(round(n * 0.375)) + 1, int(round(n * 0.625))
     ct = box_cells(n, n, n, [(lo, hi, lo, hi, lo, hi)])
     p = config_params(re=400.0, k_ratio=100.0)
+    p.t_wall = 0.0
     solid = ct.reshape(-1) == 1
     r = splitmix64(4, int(solid.sum()))
     q = np.zeros(n ** 3)

@@ -171,6 +171,50 @@ class Context:
         self._call("pressure_solve", abi.dptr(ba), abi.dptr(x), C.byref(st))
         return x.reshape(abi.padded_shape("b", self.nx, self.ny, self.nz)), st.as_dict()
 
+    # -- energy equation on the frozen flow and its adjoint ----------------------
+    # Cell vectors are (nz, ny, nx) float64 numpy arrays (host memory).
+    def energy_solve(self, tol: float = 1e-12, maxit: int = 500) -> dict:
+        info = abi.SolveInfo()
+        self._call("energy_solve", C.c_double(tol), maxit, C.byref(info))
+        return info.as_dict()
+
+    def adjoint_energy(self, dJdT: np.ndarray, tol: float = 1e-12, maxit: int = 500):
+        g = np.ascontiguousarray(dJdT, dtype=np.float64).reshape(-1)
+        dq = np.empty_like(g)
+        dk = np.empty_like(g)
+        info = abi.SolveInfo()
+        self._call("adjoint_energy", g.ctypes.data, dq.ctypes.data, dk.ctypes.data, C.c_double(tol), maxit,
+                   abi.HS_MEM_HOST, C.byref(info))
+        shp = (self.nz, self.ny, self.nx)
+        return dq.reshape(shp), dk.reshape(shp), info.as_dict()
+
+    def energy_solve_rhs(self, b: np.ndarray, transpose: bool = False, tol: float = 1e-12, maxit: int = 500):
+        ba = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        x = np.empty_like(ba)
+        info = abi.SolveInfo()
+        self._call("energy_solve_rhs", 1 if transpose else 0, ba.ctypes.data, x.ctypes.data, C.c_double(tol), maxit,
+                   abi.HS_MEM_HOST, C.byref(info))
+        return x.reshape(self.nz, self.ny, self.nx), info.as_dict()
+
+    def energy_apply(self, x: np.ndarray, transpose: bool = False) -> np.ndarray:
+        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.empty_like(xa)
+        self._call("energy_apply", 1 if transpose else 0, xa.ctypes.data, y.ctypes.data, abi.HS_MEM_HOST)
+        return y.reshape(self.nz, self.ny, self.nx)
+
+    def cells(self, name: str) -> np.ndarray:
+        which = abi.ARRAY_NAMES.index(name)
+        out = np.empty(self.nx * self.ny * self.nz)
+        self._call("get_cells", which, out.ctypes.data, abi.HS_MEM_HOST)
+        return out.reshape(self.nz, self.ny, self.nx)
+
+    def put_cells(self, name: str, arr: np.ndarray):
+        which = abi.ARRAY_NAMES.index(name)
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        if a.size != self.nx * self.ny * self.nz:
+            raise ValueError(f"{name}: expected {self.nx * self.ny * self.nz} cells, got {a.size}")
+        self._call("put_cells", which, a.ctypes.data, abi.HS_MEM_HOST)
+
 
 _LIB = None
 

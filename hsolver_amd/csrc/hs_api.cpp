@@ -425,6 +425,7 @@ void hs_destroy(hs_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     gmg_free(c);
+    energy_free(c);
     double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
                       c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->ks, c->kx, c->kz,
                       c->kq, c->kb, c->dres, c->partials};
@@ -686,6 +687,112 @@ int hs_set_source(hs_ctx* c, const double* q) {
 int hs_set_conductivity(hs_ctx* c, const double* k) {
     if (!c || !k) return HS_E_ARG;
     return hs_put_array(c, HS_A_K, k, c->d.nC);
+}
+
+/* ---- energy equation on a frozen flow and its adjoint (no reference counterpart) ---- */
+namespace {
+bool mem_ok(int32_t mem) { return mem == HS_MEM_HOST || mem == HS_MEM_DEVICE; }
+// device copy of a caller vector of nC doubles: the pointer itself when it is device
+// memory, else the staging buffer `stage` after an upload
+int stage_in(hs_ctx* c, const double* src, int32_t mem, double* stage, const double** dev) {
+    if (mem == HS_MEM_DEVICE) { *dev = src; return HS_OK; }
+    HS_CHECK(hipMemcpyAsync(stage, src, c->d.nC * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    *dev = stage;
+    return HS_OK;
+}
+int stage_out(hs_ctx* c, const double* dev, double* dst, int32_t mem) {
+    if (mem == HS_MEM_DEVICE || !dst) return HS_OK;
+    HS_CHECK(hipMemcpyAsync(dst, dev, c->d.nC * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return HS_OK;
+}
+}  // namespace
+
+int hs_energy_solve(hs_ctx* c, double tol, int32_t maxit, hs_solve_info* info) {
+    if (!c) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    return energy_solve(c, tol, maxit, info);
+}
+
+int hs_adjoint_energy(hs_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol, int32_t maxit,
+                      int32_t mem, hs_solve_info* info) {
+    if (!c || !dJdT || !mem_ok(mem)) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    double *s1, *s2, *s3;
+    HS_TRY(energy_stage(c, &s1, &s2, &s3));
+    const double* g;
+    HS_TRY(stage_in(c, dJdT, mem, s1, &g));
+    double* oq = (mem == HS_MEM_DEVICE) ? dJdq : (dJdq ? s2 : nullptr);
+    double* ok = (mem == HS_MEM_DEVICE) ? dJdk : (dJdk ? s3 : nullptr);
+    HS_TRY(energy_adjoint(c, g, oq, ok, tol, maxit, info));
+    if (dJdq) HS_TRY(stage_out(c, oq, dJdq, mem));
+    if (dJdk) HS_TRY(stage_out(c, ok, dJdk, mem));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_energy_solve_rhs(hs_ctx* c, int32_t transpose, const double* b, double* x, double tol, int32_t maxit,
+                        int32_t mem, hs_solve_info* info) {
+    if (!c || !b || !x || !mem_ok(mem)) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    double *s1, *s2;
+    HS_TRY(energy_stage(c, &s1, &s2, nullptr));
+    const double* bd;
+    HS_TRY(stage_in(c, b, mem, s1, &bd));
+    double* xd = (mem == HS_MEM_DEVICE) ? x : s2;
+    HS_TRY(energy_solve_rhs(c, transpose, bd, xd, tol, maxit, info));
+    HS_TRY(stage_out(c, xd, x, mem));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_energy_apply(hs_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem) {
+    if (!c || !x || !y || !mem_ok(mem)) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    double *s1, *s2;
+    HS_TRY(energy_stage(c, &s1, &s2, nullptr));
+    const double* xd;
+    HS_TRY(stage_in(c, x, mem, s1, &xd));
+    double* yd = (mem == HS_MEM_DEVICE) ? y : s2;
+    HS_TRY(energy_apply(c, transpose, xd, yd));
+    HS_TRY(stage_out(c, yd, y, mem));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_get_cells(hs_ctx* c, int32_t which, double* dst, int32_t mem) {
+    if (!c || !dst || !mem_ok(mem)) return HS_E_ARG;
+    const hipMemcpyKind kind = mem == HS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (which == HS_A_Q || which == HS_A_K) {
+        HS_CHECK(hipMemcpyAsync(dst, which == HS_A_Q ? c->q : c->k, c->d.nC * sizeof(double), kind, c->stream));
+    } else if (which == HS_A_TF) {
+        double* s1;
+        HS_TRY(energy_stage(c, &s1, nullptr, nullptr));
+        double* out = mem == HS_MEM_DEVICE ? dst : s1;
+        HS_TRY(cells_from_padded(c, c->tf, out));
+        HS_TRY(stage_out(c, out, dst, mem));
+    } else {
+        return HS_E_ARG;
+    }
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem) {
+    if (!c || !src || !mem_ok(mem)) return HS_E_ARG;
+    const hipMemcpyKind kind = mem == HS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (which == HS_A_Q || which == HS_A_K) {
+        HS_CHECK(hipMemcpyAsync(which == HS_A_Q ? c->q : c->k, src, c->d.nC * sizeof(double), kind, c->stream));
+    } else if (which == HS_A_TF) {
+        double* s1;
+        HS_TRY(energy_stage(c, &s1, nullptr, nullptr));
+        const double* in;
+        HS_TRY(stage_in(c, src, mem, s1, &in));
+        HS_TRY(cells_to_padded(c, in, c->tf));
+    } else {
+        return HS_E_ARG;
+    }
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    return HS_OK;
 }
 
 /* applyCellBoundaryCondition fluid_solver.h:324-405 on its own (tests / host drivers). */

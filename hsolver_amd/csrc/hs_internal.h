@@ -108,6 +108,42 @@ struct MGLevel {
     void *cw, *cs, *cb, *g, *x, *b;
 };
 
+// Control block of the energy-equation BiCGStab (device resident, see hs_energy.hip).
+struct BiCtl {
+    double bb, rr, rho, alpha, omega, beta, eps, ts, tt, ss;
+    double rhn;        // <r^, r^> of the current shadow residual
+    double red[4];
+    int iters, maxit, done, converged, breakdown, half;
+    int skip;          // 1 once the rest of the iteration must not run (done, half step, restart)
+    int restart;       // 1: the next iteration starts with r^ = r (after a (near) breakdown)
+    int restarts;      // restarts so far
+    int wbrk;          // omega could not be formed this iteration
+};
+
+// One level of the energy-equation hierarchy: face conductances d* and mass fluxes f*
+// on the min face of every cell (padded layout of the level), iterate x, right-hand side b.
+struct ELevel {
+    int nx, ny, nz;
+    long sy, sz, n;
+    void *dx, *dy, *dz, *fx, *fy, *fz, *x, *b;
+    bool own_faces;    // false when the faces alias the FP64 fine operator
+};
+
+struct EnergyState {
+    bool f32 = false;
+    std::vector<ELevel*> lv;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;  // FP64 fine operator
+    double* rhs = nullptr;                     // assembled right-hand side (P layout)
+    double *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *t = nullptr,
+           *ph = nullptr, *sh = nullptr;       // BiCGStab vectors (P layout)
+    double* stage = nullptr;                   // nx*ny*nz staging for host buffers
+    double* stage2 = nullptr;
+    double* stage3 = nullptr;
+    BiCtl* ctl = nullptr;
+    BiCtl* h_ctl = nullptr;                    // pinned mirror
+    int last_iters = 0;
+};
+
 struct hs_ctx {
     hs_params prm;
     Dims d;
@@ -153,6 +189,8 @@ struct hs_ctx {
 
     // multigrid hierarchy (created on demand)
     std::vector<MGLevel*> mg;
+    // energy equation solve / adjoint state (created on demand)
+    EnergyState* en = nullptr;
     int last_iters = 0;
 
     // live per-kernel timing: HIP events around every launch of one chosen kernel
@@ -162,11 +200,6 @@ struct hs_ctx {
     long prof_count = 0;
     double prof_ms = 0.0;
 };
-
-// kernel ids for hs_prof_set
-#define HS_K_MG_SMOOTH0  1   /* one red-black colour pass on the finest multigrid level */
-#define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
-#define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
 
 int prof_begin(hs_ctx* c, int kernel);   // returns slot or -1
 int prof_end(hs_ctx* c, int slot);
@@ -215,3 +248,13 @@ int energy_balance(hs_ctx* c, double* heat, double* resid);
 int gmg_setup(hs_ctx* c);
 int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
 void gmg_free(hs_ctx* c);
+// energy equation on the frozen flow (hs_energy.hip); vectors in the C layout are device pointers
+int energy_solve(hs_ctx* c, double tol, int maxit, hs_solve_info* info);
+int energy_adjoint(hs_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol, int maxit,
+                   hs_solve_info* info);
+int energy_apply(hs_ctx* c, int transpose, const double* x, double* y);
+int energy_solve_rhs(hs_ctx* c, int transpose, const double* b, double* x, double tol, int maxit, hs_solve_info* info);
+void energy_free(hs_ctx* c);
+int energy_stage(hs_ctx* c, double** s1, double** s2, double** s3);   // nx*ny*nz device staging
+int cells_from_padded(hs_ctx* c, const double* P, double* C);        // interior of a P array -> C layout
+int cells_to_padded(hs_ctx* c, const double* C, double* P);          // C layout -> interior of a P array

@@ -202,11 +202,52 @@ int hs_put_fields(hs_ctx* c, const double* u, const double* v, const double* w,
 int hs_set_source(hs_ctx* c, const double* q);
 int hs_set_conductivity(hs_ctx* c, const double* k);
 
+/* ---- energy equation on a frozen flow and its discrete adjoint (Mode C) ----
+ * No counterpart in the reference: its temperatures only see under-relaxed sweeps
+ * (solveTemp fluid_solver.h:794-881, MobSolver::solveTemperature mobs_solver.h:467-500),
+ * whose fixed point is the linear system solved here.  Unknowns: every interior cell
+ * (fluid and solid).  Operator: the conjugate energy stencil of the Mode C sweeps
+ * (harmonic-mean face conductance, upwind convection with the current face velocities)
+ * with the zero-gradient outflow ghost folded into the diagonal; right-hand side:
+ * q V plus the Dirichlet ghost temperatures of inlet and walls.  Solver: BiCGStab in
+ * FP64 preconditioned by a geometric multigrid V-cycle (FP32 when
+ * params.precision == HS_PREC_MIXED) built from the same face data; the transpose
+ * solve reuses the hierarchy with the convective fluxes reversed.
+ * Vectors below are cell arrays in the C layout (nx*ny*nz, x fastest); `mem` says
+ * whether pointers are host memory or device memory of the context's GPU. */
+#define HS_MEM_HOST    0
+#define HS_MEM_DEVICE  1
+typedef struct {
+    int32_t iters;                    /* BiCGStab iterations */
+    int32_t converged;                /* 1 if ||r|| <= tol ||b|| was met */
+    int32_t breakdown;                /* 1 if a BiCGStab scalar vanished and restarts ran out */
+    int32_t restarts;                 /* restarts with the current residual as shadow vector */
+    double  relres;                   /* recurrence ||r|| / ||b|| at exit */
+    double  true_relres;              /* ||b - A x|| / ||b|| recomputed in FP64 */
+} hs_solve_info;
+/* Solve the energy equation on the current flow, q and k; the result replaces the
+ * temperature field (HS_A_TF), outflow ghost included. */
+int hs_energy_solve(hs_ctx* c, double tol, int32_t maxit, hs_solve_info* info);
+/* Discrete adjoint at the current (solved) temperature: solves A^T lambda = dJdT and
+ * returns dJ/dq = lambda V and dJ/dk (all cells; FP64).  Either output may be NULL. */
+int hs_adjoint_energy(hs_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol,
+                      int32_t maxit, int32_t mem, hs_solve_info* info);
+/* A x = b (transpose = 0) or A^T x = b (transpose = 1) for a caller right-hand side b
+ * with the current operator (tangent / adjoint sensitivities); x0 = 0. */
+int hs_energy_solve_rhs(hs_ctx* c, int32_t transpose, const double* b, double* x, double tol,
+                        int32_t maxit, int32_t mem, hs_solve_info* info);
+/* y = A x (transpose = 0) or A^T x (transpose = 1) of the energy operator. */
+int hs_energy_apply(hs_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem);
+/* Cell arrays in the C layout: HS_A_TF (temperature, interior), HS_A_Q, HS_A_K. */
+int hs_get_cells(hs_ctx* c, int32_t which, double* dst, int32_t mem);
+int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem);
+
 /* ---- live kernel timing (no counterpart in the reference; its only timer is clock()
  *      around outer iterations, solver.h:53-54,156-158) ---- */
 #define HS_K_MG_SMOOTH0  1   /* one red-black colour pass on the finest multigrid level */
 #define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
 #define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
+#define HS_K_E_SMOOTH0   4   /* one red-black colour pass on the finest energy multigrid level */
 /* record HIP events on the context's stream around every launch of one kernel (0: off) */
 int hs_prof_set(hs_ctx* c, int32_t kernel);
 /* launches timed and their summed duration in ms since hs_prof_set */

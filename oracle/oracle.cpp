@@ -911,6 +911,151 @@ long arr_size(const Orc& o, int which) {
 
 struct or_ctx { Orc o; };
 
+/* ------------------------------------------- energy equation as a system ---- */
+
+/* CSR over the interior cells (C layout rows and columns). */
+struct Csr {
+    std::vector<long> rp, ci;
+    std::vector<double> v, b;
+};
+
+/* Mode C energy operator on the frozen flow: rows are the sweep's point equations
+ * ap T_P = sum a_nb T_nb + q V (temp_coeffs_cfg).  Neighbours outside the domain are
+ * ghosts: Dirichlet ghosts move a_nb T_ghost to the right-hand side; the outflow ghost
+ * (x = max when outflow is on) equals T_P at the sweep's fixed point, so its a_nb
+ * leaves the diagonal instead. */
+void energy_csr(const Orc& o, Csr& A) {
+    const long n = o.NC_();
+    A.rp.assign(n + 1, 0); A.ci.clear(); A.v.clear(); A.b.assign(n, 0.0);
+    const long sY = o.nx + 2, sZ = (long)(o.nx + 2) * (o.ny + 2);
+    for (long kk = 1; kk <= o.nz; ++kk)
+        for (long j = 1; j <= o.ny; ++j)
+            for (long i = 1; i <= o.nx; ++i) {
+                const long row = o.IC(i - 1, j - 1, kk - 1);
+                double a[6], bs;
+                double ap = temp_coeffs_cfg(o, i, j, kk, a, &bs);
+                const long cc = o.IP(i, j, kk);
+                const long nbp[6] = {cc + 1, cc - 1, cc + sY, cc - sY, cc - sZ, cc + sZ};
+                const long nbi[6][3] = {{i + 1, j, kk}, {i - 1, j, kk}, {i, j + 1, kk},
+                                        {i, j - 1, kk}, {i, j, kk - 1}, {i, j, kk + 1}};
+                double r = bs;
+                std::vector<std::pair<long, double>> off;
+                for (int m = 0; m < 6; ++m) {
+                    const long ii = nbi[m][0], jj = nbi[m][1], k2 = nbi[m][2];
+                    if (ii < 1 || ii > o.nx || jj < 1 || jj > o.ny || k2 < 1 || k2 > o.nz) {
+                        if (m == 0 && o.prm.outflow) ap -= a[m];        // zero-gradient outflow ghost
+                        else r += a[m] * o.tf[nbp[m]];                  // Dirichlet ghost
+                        continue;
+                    }
+                    off.push_back({o.IC(ii - 1, jj - 1, k2 - 1), -a[m]});
+                }
+                A.rp[row] = (long)A.ci.size();
+                A.ci.push_back(row); A.v.push_back(ap);
+                for (auto& e : off) { A.ci.push_back(e.first); A.v.push_back(e.second); }
+                A.b[row] = r;
+            }
+    A.rp[n] = (long)A.ci.size();
+}
+
+void csr_transpose(const Csr& A, Csr& T) {
+    const long n = (long)A.rp.size() - 1;
+    T.rp.assign(n + 1, 0); T.ci.assign(A.ci.size(), 0); T.v.assign(A.v.size(), 0.0); T.b = A.b;
+    for (long e = 0; e < (long)A.ci.size(); ++e) T.rp[A.ci[e] + 1]++;
+    for (long i = 0; i < n; ++i) T.rp[i + 1] += T.rp[i];
+    std::vector<long> fill(T.rp.begin(), T.rp.end() - 1);
+    for (long r = 0; r < n; ++r)
+        for (long e = A.rp[r]; e < A.rp[r + 1]; ++e) {
+            const long dst = fill[A.ci[e]]++;
+            T.ci[dst] = r; T.v[dst] = A.v[e];
+        }
+}
+
+void csr_mv(const Csr& A, const std::vector<double>& x, std::vector<double>& y) {
+    const long n = (long)A.rp.size() - 1;
+    y.assign(n, 0.0);
+    for (long r = 0; r < n; ++r) {
+        double s = 0.0;
+        for (long e = A.rp[r]; e < A.rp[r + 1]; ++e) s += A.v[e] * x[A.ci[e]];
+        y[r] = s;
+    }
+}
+
+double vdot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* BiCGStab (van der Vorst 1992), right-preconditioned by the diagonal, x0 = 0, stop
+ * when ||r|| <= tol ||b||.  On a (near) breakdown — <r^, v> = 0, omega = 0 or
+ * |<r^, r>| <= 1e-10 ||r^|| ||r|| — it restarts with the current residual as the
+ * shadow vector (at most 8 times). */
+void bicgstab_jacobi(const Csr& A, const std::vector<double>& b, double tol, int maxit, std::vector<double>& x,
+                     hs_solve_info& info) {
+    const long n = (long)b.size();
+    std::vector<double> dinv(n, 1.0);
+    for (long r = 0; r < n; ++r)
+        for (long e = A.rp[r]; e < A.rp[r + 1]; ++e)
+            if (A.ci[e] == r && A.v[e] != 0.0) dinv[r] = 1.0 / A.v[e];
+    x.assign(n, 0.0);
+    std::vector<double> r = b, rh = b, p(n, 0.0), v(n, 0.0), ph(n), s(n), sh(n), t(n);
+    const double bb = vdot(b, b);
+    double rho = bb, rho_old = bb, rhn = bb, alpha = 1.0, omega = 1.0, rr = bb;
+    info = hs_solve_info{};
+    if (bb == 0.0) { info.converged = 1; return; }
+    const double eps = tol * tol * bb;
+    int it = 0, restarts = 0;
+    bool conv = false, brk = false, fresh = true;
+    auto restart = [&]() {
+        if (restarts >= 8 || !(rr > 0.0)) return false;
+        ++restarts;
+        rh = r; rho = rr; rhn = rr; fresh = true;
+        return true;
+    };
+    while (it < maxit) {
+        const double beta = fresh ? 0.0 : (rho / rho_old) * (alpha / omega);
+        fresh = false;
+        for (long i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        for (long i = 0; i < n; ++i) ph[i] = dinv[i] * p[i];
+        csr_mv(A, ph, v);
+        const double sig = vdot(rh, v);
+        if (sig == 0.0) { if (restart()) continue; brk = true; break; }
+        alpha = rho / sig;
+        for (long i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        ++it;
+        if (vdot(s, s) <= eps) {
+            for (long i = 0; i < n; ++i) x[i] += alpha * ph[i];
+            r = s; rr = vdot(r, r); conv = true; break;
+        }
+        for (long i = 0; i < n; ++i) sh[i] = dinv[i] * s[i];
+        csr_mv(A, sh, t);
+        const double tt = vdot(t, t), ts = vdot(t, s);
+        const bool wbrk = !(tt > 0.0) || ts == 0.0;
+        omega = wbrk ? 0.0 : ts / tt;
+        for (long i = 0; i < n; ++i) { x[i] += alpha * ph[i] + omega * sh[i]; r[i] = s[i] - omega * t[i]; }
+        rr = vdot(r, r);
+        if (rr <= eps) { conv = true; break; }
+        const double rho_new = vdot(rh, r);
+        if (wbrk || !(std::fabs(rho_new) > 1e-10 * std::sqrt(rhn * rr))) {
+            if (restart()) continue;
+            brk = true;
+            break;
+        }
+        rho_old = rho;
+        rho = rho_new;
+    }
+    std::vector<double> ax;
+    csr_mv(A, x, ax);
+    double e2 = 0.0;
+    for (long i = 0; i < n; ++i) e2 += (b[i] - ax[i]) * (b[i] - ax[i]);
+    info.iters = it;
+    info.converged = conv ? 1 : 0;
+    info.breakdown = brk ? 1 : 0;
+    info.restarts = restarts;
+    info.relres = std::sqrt(rr / bb);
+    info.true_relres = std::sqrt(e2 / bb);
+}
+
 extern "C" {
 
 int or_create(const hs_grid* g, const hs_params* p, or_ctx** out) {
@@ -1242,34 +1387,151 @@ int or_export_energy_csr(or_ctx* c, int64_t* nnz, int64_t* rowptr, int64_t* cols
     if (!c) return HS_E_ARG;
     Orc& o = c->o;
     if (o.modeR()) return HS_E_STATE;
-    long nz_ = 0;
+    Csr A;
+    energy_csr(o, A);
+    if (nnz) *nnz = (int64_t)A.ci.size();
+    if (rowptr) for (size_t i = 0; i < A.rp.size(); ++i) rowptr[i] = A.rp[i];
+    if (cols) for (size_t i = 0; i < A.ci.size(); ++i) { cols[i] = A.ci[i]; vals[i] = A.v[i]; }
+    if (rhs) for (size_t i = 0; i < A.b.size(); ++i) rhs[i] = A.b[i];
+    return HS_OK;
+}
+
+int or_energy_solve(or_ctx* c, double tol, int32_t maxit, hs_solve_info* info) {
+    if (!c) return HS_E_ARG;
+    Orc& o = c->o;
+    if (o.modeR()) return HS_E_ARG;
+    if (!o.inited) return HS_E_STATE;
+    if (!(tol > 0.0) || maxit < 1) return HS_E_ARG;
+    Csr A;
+    energy_csr(o, A);
+    std::vector<double> x;
+    hs_solve_info inf{};
+    bicgstab_jacobi(A, A.b, tol, maxit, x, inf);
+    if (info) *info = inf;
+    for (long kk = 1; kk <= o.nz; ++kk)
+        for (long j = 1; j <= o.ny; ++j)
+            for (long i = 1; i <= o.nx; ++i) o.tf[o.IP(i, j, kk)] = x[o.IC(i - 1, j - 1, kk - 1)];
+    bc_cfg_outflow(o, 3);
+    if (!std::isfinite(inf.true_relres)) return HS_E_NAN;
+    return HS_OK;
+}
+
+/* Discrete adjoint: A^T lambda = dJ/dT on the explicit transpose, then
+ *   dJ/dq_P = lambda_P V_P
+ *   dJ/dk_P = -sum_faces dD/dk_P (lambda_P - lambda_nb)(T_P - T_nb)        (interior)
+ *             -sum_bdry  (A/h_P) lambda_P (T_P - T_ghost)                   (Dirichlet)
+ * from R(T) = A(k) T - b(q, k), dJ/dp = -lambda^T dR/dp. */
+int or_adjoint_energy(or_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol, int32_t maxit,
+                      int32_t mem, hs_solve_info* info) {
+    if (!c || !dJdT) return HS_E_ARG;
+    Orc& o = c->o;
+    if (o.modeR()) return HS_E_ARG;
+    if (!o.inited) return HS_E_STATE;
+    if (mem != HS_MEM_HOST || !(tol > 0.0) || maxit < 1) return HS_E_ARG;
+    Csr A, At;
+    energy_csr(o, A);
+    csr_transpose(A, At);
+    std::vector<double> g(dJdT, dJdT + o.NC_()), lam;
+    hs_solve_info inf{};
+    bicgstab_jacobi(At, g, tol, maxit, lam, inf);
+    if (info) *info = inf;
+    const long sY = o.nx + 2, sZ = (long)(o.nx + 2) * (o.ny + 2);
     for (long kk = 1; kk <= o.nz; ++kk)
         for (long j = 1; j <= o.ny; ++j)
             for (long i = 1; i <= o.nx; ++i) {
-                const long row = o.IC(i - 1, j - 1, kk - 1);
-                double a[6], bs;
-                const double ap = temp_coeffs_cfg(o, i, j, kk, a, &bs);
-                const long sY = o.nx + 2, sZ = (long)(o.nx + 2) * (o.ny + 2);
-                const long cc = o.IP(i, j, kk);
-                const long nbp[6] = {cc + 1, cc - 1, cc + sY, cc - sY, cc - sZ, cc + sZ};
-                const long nbi[6][3] = {{i + 1, j, kk}, {i - 1, j, kk}, {i, j + 1, kk}, {i, j - 1, kk}, {i, j, kk - 1}, {i, j, kk + 1}};
-                double r = bs;
-                if (rowptr) rowptr[row] = nz_;
-                if (cols) { cols[nz_] = row; vals[nz_] = ap; }
-                ++nz_;
-                for (int m = 0; m < 6; ++m) {
-                    const long ii = nbi[m][0], jj = nbi[m][1], k2 = nbi[m][2];
-                    if (ii < 1 || ii > o.nx || jj < 1 || jj > o.ny || k2 < 1 || k2 > o.nz) {
-                        r += a[m] * o.tf[nbp[m]];  // Dirichlet / outflow ghost moved to the rhs
-                        continue;
+                const long ic = o.IC(i - 1, j - 1, kk - 1), cc = o.IP(i, j, kk);
+                const double xp = o.xp[i], yp = o.yp[j], zp = o.zp[kk];
+                const double lP = lam[ic];
+                if (dJdq) dJdq[ic] = lP * (xp * yp * zp);
+                if (!dJdk) continue;
+                const double kP = o.k[ic], TP = o.tf[cc];
+                // the six faces: area, this cell's width, neighbour (in domain?), its width, C and P offsets
+                struct Face { double A, h; bool in; double hn; long dc, dp; bool outflow; };
+                const Face f[6] = {
+                    {yp * zp, xp, i > 1, i > 1 ? o.xp[i - 1] : 0.0, -1, -1, false},
+                    {yp * zp, xp, i < o.nx, i < o.nx ? o.xp[i + 1] : 0.0, 1, 1, i == o.nx && o.prm.outflow},
+                    {xp * zp, yp, j > 1, j > 1 ? o.yp[j - 1] : 0.0, -(long)o.nx, -sY, false},
+                    {xp * zp, yp, j < o.ny, j < o.ny ? o.yp[j + 1] : 0.0, (long)o.nx, sY, false},
+                    {xp * yp, zp, kk > 1, kk > 1 ? o.zp[kk - 1] : 0.0, -(long)o.nx * o.ny, -sZ, false},
+                    {xp * yp, zp, kk < o.nz, kk < o.nz ? o.zp[kk + 1] : 0.0, (long)o.nx * o.ny, sZ, false}};
+                double sum = 0.0;
+                for (const Face& F : f) {
+                    if (F.in) {
+                        const double kN = o.k[ic + F.dc];
+                        const double den = 0.5 * F.h / kP + 0.5 * F.hn / kN;
+                        const double dD = F.A * (0.5 * F.h) / (kP * kP * den * den);
+                        sum += dD * (lP - lam[ic + F.dc]) * (TP - o.tf[cc + F.dp]);
+                    } else if (!F.outflow) {
+                        sum += (F.A / F.h) * lP * (TP - o.tf[cc + F.dp]);
                     }
-                    if (cols) { cols[nz_] = o.IC(ii - 1, jj - 1, k2 - 1); vals[nz_] = -a[m]; }
-                    ++nz_;
                 }
-                if (rhs) rhs[row] = r;
+                dJdk[ic] = -sum;
             }
-    if (rowptr) rowptr[o.NC_()] = nz_;
-    if (nnz) *nnz = nz_;
+    if (!std::isfinite(inf.true_relres)) return HS_E_NAN;
+    return HS_OK;
+}
+
+int or_energy_solve_rhs(or_ctx* c, int32_t transpose, const double* b, double* x, double tol, int32_t maxit,
+                        int32_t mem, hs_solve_info* info) {
+    if (!c || !b || !x || mem != HS_MEM_HOST) return HS_E_ARG;
+    Orc& o = c->o;
+    if (o.modeR()) return HS_E_ARG;
+    if (!o.inited) return HS_E_STATE;
+    if (!(tol > 0.0) || maxit < 1) return HS_E_ARG;
+    Csr A, At;
+    energy_csr(o, A);
+    const Csr* M = &A;
+    if (transpose) { csr_transpose(A, At); M = &At; }
+    std::vector<double> bv(b, b + o.NC_()), xv;
+    hs_solve_info inf{};
+    bicgstab_jacobi(*M, bv, tol, maxit, xv, inf);
+    if (info) *info = inf;
+    for (long i = 0; i < o.NC_(); ++i) x[i] = xv[i];
+    return std::isfinite(inf.true_relres) ? HS_OK : HS_E_NAN;
+}
+
+int or_energy_apply(or_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem) {
+    if (!c || !x || !y || mem != HS_MEM_HOST) return HS_E_ARG;
+    Orc& o = c->o;
+    if (o.modeR()) return HS_E_ARG;
+    if (!o.inited) return HS_E_STATE;
+    Csr A, At;
+    energy_csr(o, A);
+    const Csr* M = &A;
+    if (transpose) { csr_transpose(A, At); M = &At; }
+    std::vector<double> xv(x, x + o.NC_()), yv;
+    csr_mv(*M, xv, yv);
+    for (long i = 0; i < o.NC_(); ++i) y[i] = yv[i];
+    return HS_OK;
+}
+
+int or_get_cells(or_ctx* c, int32_t which, double* dst, int32_t mem) {
+    if (!c || !dst || mem != HS_MEM_HOST) return HS_E_ARG;
+    Orc& o = c->o;
+    if (which == HS_A_Q || which == HS_A_K) {
+        const std::vector<double>& s = which == HS_A_Q ? o.q : o.k;
+        for (long i = 0; i < o.NC_(); ++i) dst[i] = s[i];
+        return HS_OK;
+    }
+    if (which != HS_A_TF) return HS_E_ARG;
+    for (long kk = 1; kk <= o.nz; ++kk)
+        for (long j = 1; j <= o.ny; ++j)
+            for (long i = 1; i <= o.nx; ++i) dst[o.IC(i - 1, j - 1, kk - 1)] = o.tf[o.IP(i, j, kk)];
+    return HS_OK;
+}
+
+int or_put_cells(or_ctx* c, int32_t which, const double* src, int32_t mem) {
+    if (!c || !src || mem != HS_MEM_HOST) return HS_E_ARG;
+    Orc& o = c->o;
+    if (which == HS_A_Q || which == HS_A_K) {
+        std::vector<double>& s = which == HS_A_Q ? o.q : o.k;
+        for (long i = 0; i < o.NC_(); ++i) s[i] = src[i];
+        return HS_OK;
+    }
+    if (which != HS_A_TF) return HS_E_ARG;
+    for (long kk = 1; kk <= o.nz; ++kk)
+        for (long j = 1; j <= o.ny; ++j)
+            for (long i = 1; i <= o.nx; ++i) o.tf[o.IP(i, j, kk)] = src[o.IC(i - 1, j - 1, kk - 1)];
     return HS_OK;
 }
 

@@ -46,9 +46,22 @@ int     or_cell_bc(or_ctx* c);
 int     or_export_csr(or_ctx* c, int64_t* nrows, int64_t* nnz, int64_t* rowptr,
                       int64_t* cols, double* vals, double* rhs, int64_t* row_cell);
 /* energy-equation operator of Mode C on the current (frozen) flow:
- * A T = s over all interior cells, C layout, CSR; used by the adjoint checks. */
+ * A T = s over all interior cells, C layout, CSR, the outflow ghost folded into
+ * the diagonal (the sweep's fixed point); used by the adjoint checks. */
 int     or_export_energy_csr(or_ctx* c, int64_t* nnz, int64_t* rowptr, int64_t* cols,
                              double* vals, double* rhs);
+
+/* energy equation on the frozen flow and its discrete adjoint (see hsolver.h);
+ * FP64 BiCGStab with a diagonal preconditioner on the assembled CSR and its
+ * explicit transpose; host memory only */
+int     or_energy_solve(or_ctx* c, double tol, int32_t maxit, hs_solve_info* info);
+int     or_adjoint_energy(or_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol,
+                          int32_t maxit, int32_t mem, hs_solve_info* info);
+int     or_energy_solve_rhs(or_ctx* c, int32_t transpose, const double* b, double* x, double tol,
+                            int32_t maxit, int32_t mem, hs_solve_info* info);
+int     or_energy_apply(or_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem);
+int     or_get_cells(or_ctx* c, int32_t which, double* dst, int32_t mem);
+int     or_put_cells(or_ctx* c, int32_t which, const double* src, int32_t mem);
 
 #ifdef __cplusplus
 }

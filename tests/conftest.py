@@ -32,5 +32,8 @@ def oracle_lib():
 
 @pytest.fixture(scope="session")
 def hs_lib():
+    # torch first: its bundled HIP runtime then serves the library too (one runtime per
+    # process; loading the library first maps a second runtime that torch.cuda cannot use)
+    import torch  # noqa: F401
     from hsolver_amd import core
     return core.library()

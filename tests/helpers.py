@@ -1,5 +1,8 @@
 """Shared test helpers: oracle / GPU contexts on the same case, small grids."""
+import ctypes as C
+
 import numpy as np
+import scipy.sparse as sp
 
 from hsolver_amd import abi, configs
 from hsolver_amd.core import Context
@@ -37,3 +40,34 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64).ravel()
     den = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def export_energy(ctx, n):
+    """The oracle's Mode C energy operator on the current flow as scipy CSR + rhs."""
+    lib = ctx.lib
+    I64 = C.c_int64
+    lib.or_export_energy_csr.argtypes = [C.c_void_p, C.POINTER(I64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    nz = I64()
+    assert lib.or_export_energy_csr(ctx.h, C.byref(nz), None, None, None, None) == 0
+    rp = np.zeros(n + 1, np.int64)
+    cols = np.zeros(nz.value, np.int64)
+    vals = np.zeros(nz.value)
+    rhs = np.zeros(n)
+    assert lib.or_export_energy_csr(ctx.h, C.byref(nz), rp.ctypes.data, cols.ctypes.data, vals.ctypes.data,
+                                    rhs.ctypes.data) == 0
+    return sp.csr_matrix((vals, cols, rp), shape=(n, n)), rhs
+
+
+def solid_pnorm(T, solid, V, p=8):
+    """J = (sum over solid cells of T^p V)^(1/p) and dJ/dT (numpy, FP64)."""
+    T = np.asarray(T, dtype=np.float64)
+    S = np.sum(np.where(solid, T ** p * V, 0.0))
+    J = S ** (1.0 / p)
+    g = np.where(solid, J ** (1 - p) * T ** (p - 1) * V, 0.0)
+    return J, g
+
+
+def cell_volumes(case):
+    g = case.grid
+    dx, dy, dz = np.diff(g.xv), np.diff(g.yv), np.diff(g.zv)
+    return dz[:, None, None] * dy[None, :, None] * dx[None, None, :]

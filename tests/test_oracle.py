@@ -19,7 +19,7 @@ import scipy.sparse.linalg as spla
 
 from hsolver_amd import abi, configs
 
-from helpers import oracle_ctx, small_ref_case
+from helpers import export_energy, oracle_ctx, small_ref_case
 
 
 def uniform_case(n=6, mode=abi.HS_MODE_REF, h=1.0 / 6, block=None):
@@ -47,22 +47,6 @@ def export_csr(ctx):
     assert lib.or_export_csr(h, C.byref(nr), C.byref(nz), rp.ctypes.data, cols.ctypes.data, vals.ctypes.data,
                              rhs.ctypes.data, cell.ctypes.data) == 0
     return sp.csr_matrix((vals, cols, rp), shape=(nr.value, nr.value)), rhs, cell
-
-
-def export_energy(ctx, n):
-    import ctypes as C
-    lib = ctx.lib
-    I64 = C.c_int64
-    lib.or_export_energy_csr.argtypes = [C.c_void_p, C.POINTER(I64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-    nz = I64()
-    assert lib.or_export_energy_csr(ctx.h, C.byref(nz), None, None, None, None) == 0
-    rp = np.zeros(n + 1, np.int64)
-    cols = np.zeros(nz.value, np.int64)
-    vals = np.zeros(nz.value)
-    rhs = np.zeros(n)
-    assert lib.or_export_energy_csr(ctx.h, C.byref(nz), rp.ctypes.data, cols.ctypes.data, vals.ctypes.data,
-                                    rhs.ctypes.data) == 0
-    return sp.csr_matrix((vals, cols, rp), shape=(n, n)), rhs
 
 
 # ------------------------------------------------------------------ momentum ---
