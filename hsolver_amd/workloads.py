@@ -1,0 +1,119 @@
+"""The benchmark configurations other than the headline one, run on the GPU.
+
+    python -m hsolver_amd.workloads c1 [--iters 200]
+    python -m hsolver_amd.workloads c2 [--iters 500]
+
+c1  64x16x16 channel, solid cube, Re 100, k ratio 100, q = 1: the reference's solver
+    settings (lexicographic sweeps, unpreconditioned CG to 1e-8) and the fast path
+    (red-black sweeps, multigrid FCG), SIMPLE iterations/s and balance residuals.
+c2  256x64x64 channel (1.05M cells), heated cube with seeded q, Re 400: the FP64 drop-in
+    (red-black sweeps, FP64 unpreconditioned CG = the reference's solver) and FP64
+    multigrid FCG; SIMPLE iterations/s and the HBM rate of the p' SpMV and of one
+    x-momentum colour pass, timed live with HIP events around every launch:
+      SpMV (k_cg_spmv): 41 B per cell (x, three face-coefficient arrays, q written, class)
+      momentum colour pass (k_mom_rb<0>): 41 B per x-face (u, v, w, p_temp read,
+      u and a_P written on the colour, class)
+Each prints one JSON line.  Inputs are synthetic (configs.py), resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import sys
+import time
+
+from . import abi, configs, core
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[workloads] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_run(case, order, precond, precision, iters, prof_kernel=0):
+    lib = core.library()
+    lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
+    lib.hs_prof_get.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    p = abi.Params.from_buffer_copy(case.params)
+    p.order, p.precond, p.precision = order, precond, precision
+    ctx = core.create(case.grid, p, case.q)
+    ctx.init()
+    ctx.step()                                   # warm-up (allocations, hierarchy)
+    if prof_kernel:
+        lib.hs_prof_set(ctx.h, prof_kernel)
+    hist = []
+    t0 = time.perf_counter()
+    for s in range(iters):
+        hist.append(ctx.step())
+        if (s + 1) % 100 == 0:
+            log(f"{case.name}: {s + 1}/{iters} iterations, mass imbalance {hist[-1]['mass_imbalance']:.3e}")
+    dt = time.perf_counter() - t0
+    prof = None
+    if prof_kernel:
+        n, ms = C.c_int64(), C.c_double()
+        lib.hs_prof_get(ctx.h, C.byref(n), C.byref(ms))
+        prof = (n.value, ms.value)
+        lib.hs_prof_set(ctx.h, 0)
+    ctx.close()
+    return dt, hist, prof
+
+
+def summary(dt, hist, iters):
+    last = hist[-1]
+    return {"simple_iters_per_s": round(iters / dt, 3), "ms_per_iter": round(1e3 * dt / iters, 3),
+            "p_iters_last": last["p_iters"], "p_iters_mean": round(sum(h["p_iters"] for h in hist) / len(hist), 1),
+            "mass_imbalance": last["mass_imbalance"], "energy_residual": last["energy_residual"],
+            "p_true_relres_max": max(h["p_true_relres"] for h in hist)}
+
+
+def rate(prof, bytes_per_launch):
+    n, ms = prof
+    avg = ms / max(n, 1)
+    gbs = bytes_per_launch / (avg * 1e-3) / 1e9
+    return {"launches": n, "avg_us": round(avg * 1e3, 2), "GB_s": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def run_c1(iters):
+    case = configs.c1()
+    out = {"workload": "c1 channel 64x16x16, solid cube, Re=100, k ratio 100", "iters": iters}
+    dt, hist, _ = timed_run(case, abi.HS_ORDER_LEX, abi.HS_PRECOND_NONE, abi.HS_PREC_FP64, iters)
+    out["reference_settings"] = summary(dt, hist, iters)
+    for name, prec in (("fast_fp64", abi.HS_PREC_FP64), ("fast_mixed", abi.HS_PREC_MIXED)):
+        dt, hist, _ = timed_run(case, abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, prec, iters)
+        out[name] = summary(dt, hist, iters)
+    return out
+
+
+def run_c2(iters):
+    case = configs.c2()
+    nx, ny, nz = case.shape
+    ncell = nx * ny * nz
+    out = {"workload": f"c2 channel {nx}x{ny}x{nz}, heated cube, Re=400", "iters": iters}
+    dt, hist, prof = timed_run(case, abi.HS_ORDER_RB, abi.HS_PRECOND_NONE, abi.HS_PREC_FP64, iters,
+                               prof_kernel=2)
+    out["fp64_dropin_cg"] = summary(dt, hist, iters)
+    out["fp64_dropin_cg"]["spmv"] = rate(prof, 41.0 * ncell)
+    dt, hist, prof = timed_run(case, abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, abi.HS_PREC_FP64, iters,
+                               prof_kernel=3)
+    out["fp64_gmg"] = summary(dt, hist, iters)
+    out["fp64_gmg"]["momentum_colour_pass"] = rate(prof, 41.0 * (nx + 1) * ny * nz)
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("config", choices=["c1", "c2"])
+    ap.add_argument("--iters", type=int, default=None)
+    args = ap.parse_args(argv)
+    if args.config == "c1":
+        out = run_c1(args.iters or 200)
+    else:
+        out = run_c2(args.iters or 500)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

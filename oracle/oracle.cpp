@@ -38,6 +38,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -1006,6 +1008,7 @@ void bicgstab_jacobi(const Csr& A, const std::vector<double>& b, double tol, int
     const double eps = tol * tol * bb;
     int it = 0, restarts = 0;
     bool conv = false, brk = false, fresh = true;
+    const bool progress = std::getenv("HS_ORACLE_PROGRESS") != nullptr;   // long solves report as they go
     auto restart = [&]() {
         if (restarts >= 8 || !(rr > 0.0)) return false;
         ++restarts;
@@ -1034,6 +1037,7 @@ void bicgstab_jacobi(const Csr& A, const std::vector<double>& b, double tol, int
         omega = wbrk ? 0.0 : ts / tt;
         for (long i = 0; i < n; ++i) { x[i] += alpha * ph[i] + omega * sh[i]; r[i] = s[i] - omega * t[i]; }
         rr = vdot(r, r);
+        if (progress && it % 100 == 0) std::fprintf(stderr, "[oracle] bicgstab %d: relres %.3e\n", it, std::sqrt(rr / bb));
         if (rr <= eps) { conv = true; break; }
         const double rho_new = vdot(rh, r);
         if (wbrk || !(std::fabs(rho_new) > 1e-10 * std::sqrt(rhn * rr))) {

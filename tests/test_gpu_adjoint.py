@@ -131,3 +131,52 @@ def test_autograd_function(pair):
     dq, dk, _ = g.adjoint_energy(dJdT, tol=TOL, maxit=300)
     assert rel_l2(q.grad.cpu().numpy(), dq) < 1e-10
     assert rel_l2(k.grad.cpu().numpy(), dk) < 1e-10
+
+
+@pytest.mark.skipif(__import__("os").environ.get("HS_SLOW") != "1",
+                    reason="the 128^3 oracle solves take minutes on the CPU; run with HS_SLOW=1")
+def test_c4_128_full_gradient_vs_oracle(hs_lib, oracle_lib):
+    """The adjoint workload at full size: converged 128^3 flow (mixed-precision SIMPLE),
+    frozen; forward solve and full dJ/dq, dJ/dk of both GPU precisions against the
+    oracle's FP64 solves on the same state."""
+    import os
+    from hsolver_amd.adjoint import converge_flow
+    os.environ["HS_ORACLE_PROGRESS"] = "1"
+    n = 128
+    case = configs.c4(n)
+    p = abi.Params.from_buffer_copy(case.params)
+    p.order = abi.HS_ORDER_RB
+    p.precond = abi.HS_PRECOND_GMG
+    p.precision = abi.HS_PREC_MIXED
+    g = gpu_ctx(hs_lib, case, p)
+    g.init()
+    hist = converge_flow(g, 3000, 1e-9, log=lambda m: print(f"[adjoint] {m}", flush=True))
+    print(f"[adjoint] flow: {len(hist)} iterations, mass imbalance {hist[-1]:.3e}", flush=True)
+    state = g.state()
+    o = oracle_ctx(oracle_lib, case)
+    o.init()
+    o.load_state(state)
+    io = o.energy_solve(tol=TOL, maxit=20000)
+    assert io["converged"] == 1, io
+    To = o.cells("tf")
+    solid = case.grid.cell_type.reshape(n, n, n) == 1
+    _, dJdT = solid_pnorm(To, solid, cell_volumes(case))
+    dq_o, dk_o, info_o = o.adjoint_energy(dJdT, tol=TOL, maxit=20000)
+    assert info_o["converged"] == 1, info_o
+    for prec in (abi.HS_PREC_MIXED, abi.HS_PREC_FP64):
+        p.precision = prec
+        h = gpu_ctx(hs_lib, case, p)
+        h.init()
+        h.load_state(state)
+        ig = h.energy_solve(tol=TOL, maxit=400)
+        assert ig["converged"] == 1, ig
+        eT = rel_l2(h.cells("tf"), To)
+        dq, dk, ia = h.adjoint_energy(dJdT, tol=TOL, maxit=400)
+        assert ia["converged"] == 1, ia
+        eq, ek = rel_l2(dq, dq_o), rel_l2(dk, dk_o)
+        print(f"[adjoint] 128^3 precision {prec}: forward {ig['iters']} it, adjoint {ia['iters']} it; "
+              f"rel-L2 T {eT:.2e}, dJ/dq {eq:.2e}, dJ/dk {ek:.2e}", flush=True)
+        assert eT < 1e-10 and eq < 1e-9 and ek < 1e-8
+        h.close()
+    g.close()
+    o.close()
