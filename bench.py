@@ -4,7 +4,9 @@
 Workload (the metric's configuration): 256^3 unit box with 27 seeded solid blocks,
 conductivity ratio 1000, lid + inlet forcing at Re = 1000, configuration physics,
 red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual.
-  value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG)
+  value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG; the
+               16^3 coarsest level solved with a dense FP32 inverse built on the matrix
+               cores plus FP64 refinement, rebuilt every --dense hierarchy builds)
   fp64_value = the same step with the V-cycle in FP64
 Each timed step is one full SIMPLE outer iteration (x/y/z momentum, p' assembly +
 solve, velocity / pressure correction, conjugate energy sweep, set-for-next), all
@@ -42,7 +44,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None):
+def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0):
     from hsolver_amd import abi, core
     lib = core.library()
     lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
@@ -51,6 +53,7 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
     p.precision = precision
     p.precond = precond
     p.order = abi.HS_ORDER_RB if order is None else order
+    p.mg_dense_coarse = dense
     ctx = core.create(case.grid, p, case.q)
     ctx.init()
     for w in range(warmup):
@@ -123,6 +126,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 multigrid comparison run")
     ap.add_argument("--no-cg", action="store_true", help="skip the unpreconditioned FP64 CG drop-in step")
+    ap.add_argument("--dense", type=int, default=10, metavar="N",
+                    help="mixed path: coarsest level (16^3) solved with the dense FP32 inverse built on the "
+                         "matrix cores, rebuilt every N hierarchy builds (0: red-black sweeps there)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
@@ -150,7 +156,7 @@ def main() -> int:
         f"(built in {time.perf_counter() - t_case:.1f} s)")
 
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                       prof_kernel=1)
+                                       prof_kernel=1, dense=args.dense)
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
@@ -198,6 +204,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
                    "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG V(2,2), RB-GS",
+                   "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt every {args.dense} builds, "
+                                    "2 FP64 refinement steps" if args.dense else "single-workgroup RB-GS tail"),
+                   "coarse_fp64": "single-workgroup RB-GS tail",
                    "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},
         "fp64_value": round(fp64_value, 4) if fp64_value else None,
         "mixed_over_fp64": round(value / fp64_value, 4) if fp64_value else None,

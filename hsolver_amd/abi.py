@@ -44,7 +44,8 @@ class Params(C.Structure):
                 ("mg_levels", C.c_int32), ("mg_pre", C.c_int32), ("mg_post", C.c_int32),
                 ("mg_coarse_sweeps", C.c_int32), ("mg_dense_coarse", C.c_int32),
                 ("t_fluid_init", C.c_double), ("t_solid_init", C.c_double), ("t_wall", C.c_double),
-                ("fail_on_noconv", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -107,6 +108,7 @@ def params_default(mode: int) -> Params:
     p.mg_post = 2
     p.mg_coarse_sweeps = 40
     p.mg_dense_coarse = 0
+    p.mg_dense_refine = 2
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0
     p.t_wall = 293.0

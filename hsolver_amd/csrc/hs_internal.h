@@ -144,6 +144,17 @@ struct EnergyState {
     int last_iters = 0;
 };
 
+// Dense inverse of the coarsest multigrid level (hs_dense.hip).
+struct DenseCoarse {
+    int np = 0;                    // order, padded to a multiple of 64
+    float* M = nullptr;            // np x np, row-major: the level matrix, then its inverse
+    float* C = nullptr;            // np x 64 panel of the Gauss-Jordan step
+    float* P = nullptr;            // 64 x 64 pivot-block inverse
+    double *r = nullptr, *z = nullptr;   // right-hand side, solution (FP64)
+    float* xf = nullptr;                 // FP32 input of the inverse apply
+    long builds = 0, factorizations = 0;
+};
+
 struct hs_ctx {
     hs_params prm;
     Dims d;
@@ -191,6 +202,8 @@ struct hs_ctx {
     std::vector<MGLevel*> mg;
     // energy equation solve / adjoint state (created on demand)
     EnergyState* en = nullptr;
+    // dense coarsest-level inverse (created on demand)
+    DenseCoarse* dense = nullptr;
     int last_iters = 0;
 
     // live per-kernel timing: HIP events around every launch of one chosen kernel
@@ -248,6 +261,11 @@ int energy_balance(hs_ctx* c, double* heat, double* resid);
 int gmg_setup(hs_ctx* c);
 int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
 void gmg_free(hs_ctx* c);
+// dense coarsest-level solve (hs_dense.hip)
+int dense_max_cells();
+int dense_setup(hs_ctx* c, const MGLevel* m, bool f32);
+int dense_solve(hs_ctx* c, const MGLevel* m, bool f32);
+void dense_free(hs_ctx* c);
 // energy equation on the frozen flow (hs_energy.hip); vectors in the C layout are device pointers
 int energy_solve(hs_ctx* c, double tol, int maxit, hs_solve_info* info);
 int energy_adjoint(hs_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol, int maxit,

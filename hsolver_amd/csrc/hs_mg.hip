@@ -26,6 +26,7 @@
 #include <cmath>
 
 #include "hs_device.h"
+#include "hs_mglevel.h"
 
 using hsd::BT;
 using hsd::BX;
@@ -35,17 +36,10 @@ using hsd::FLAT_BLOCKS;
 
 namespace {
 
-template <typename T>
-struct Lv {
-    int nx, ny, nz;
-    long sy, sz;
-    T *cw, *cs, *cb, *g, *x, *b;
-};
-
-template <typename T>
-Lv<T> lv(const MGLevel* m) {
-    return Lv<T>{m->nx, m->ny, m->nz, m->sy, m->sz, (T*)m->cw, (T*)m->cs, (T*)m->cb, (T*)m->g, (T*)m->x, (T*)m->b};
-}
+using mgl::Lv;
+using mgl::lv;
+using mgl::diag_of;
+using mgl::offsum;
 
 // fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask
 template <typename T>
@@ -121,18 +115,6 @@ __global__ void k_mg_coarsen(Lv<T> F, Lv<T> C, T gscale) {
     C.cs[cc] = ss;
     C.cb[cc] = sb;
     C.g[cc] = gscale * g;
-}
-
-template <typename T>
-__device__ __forceinline__ T diag_of(const Lv<T>& L, long c) {
-    return L.g[c] + L.cw[c] + L.cw[c + 1] + L.cs[c] + L.cs[c + L.sy] + L.cb[c] + L.cb[c + L.sz];
-}
-
-template <typename T>
-__device__ __forceinline__ T offsum(const Lv<T>& L, long c) {
-    const T* x = L.x;
-    return L.cw[c] * x[c - 1] + L.cw[c + 1] * x[c + 1] + L.cs[c] * x[c - L.sy] + L.cs[c + L.sy] * x[c + L.sy] +
-           L.cb[c] * x[c - L.sz] + L.cb[c + L.sz] * x[c + L.sz];
 }
 
 // one colour of red-black Gauss-Seidel
@@ -390,6 +372,24 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
 template <typename T>
 int vcycle(hs_ctx* c) {
     const int nl = (int)c->mg.size();
+    if (c->prm.mg_dense_coarse > 0 && c->dense) {
+        // every level but the coarsest smoothed by the grid kernels, the coarsest solved
+        // directly with its dense inverse (+ FP64 refinement)
+        for (int l = 0; l + 1 < nl; ++l) {
+            HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
+            Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
+            k_mg_restrict<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, c->sctl);
+            HS_CHECK(hipGetLastError());
+        }
+        HS_TRY(dense_solve(c, c->mg[nl - 1], sizeof(T) == sizeof(float)));
+        for (int l = nl - 2; l >= 0; --l) {
+            Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
+            k_mg_prolong<T><<<dim3((F.nx + 63) / 64, F.ny, F.nz), 64, 0, c->stream>>>(F, C, c->sctl);
+            HS_CHECK(hipGetLastError());
+            HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
+        }
+        return HS_OK;
+    }
     // first level small enough for the single-workgroup tail
     int t = 0;
     while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
@@ -430,6 +430,8 @@ int build_hierarchy(hs_ctx* c) {
             c->mg.push_back(m);
             HS_TRY(alloc_level<T>(m));
             if (std::min(nx, std::min(ny, nz)) <= 4) break;
+            // with the dense coarse solve the hierarchy ends at the first small enough level
+            if (c->prm.mg_dense_coarse > 0 && (long)nx * ny * nz <= dense_max_cells()) break;
             nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
         }
     }
@@ -446,6 +448,8 @@ int build_hierarchy(hs_ctx* c) {
         k_mg_coarsen<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, gscale);
         HS_CHECK(hipGetLastError());
     }
+    if (c->prm.mg_dense_coarse > 0 && (long)c->mg.back()->nx * c->mg.back()->ny * c->mg.back()->nz <= dense_max_cells())
+        HS_TRY(dense_setup(c, c->mg.back(), sizeof(T) == sizeof(float)));
     return HS_OK;
 }
 
@@ -510,6 +514,7 @@ int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
 }
 
 void gmg_free(hs_ctx* c) {
+    dense_free(c);
     for (MGLevel* m : c->mg) {
         void* arrs[6] = {m->cw, m->cs, m->cb, m->g, m->x, m->b};
         for (void* a : arrs) if (a) hipFree(a);

@@ -96,12 +96,15 @@ typedef struct {
     int32_t mg_levels;                /* multigrid levels including the finest; 0 = down to <= 4 cells a side */
     int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
-    int32_t mg_dense_coarse;          /* 1 = dense LU (MFMA) on the coarsest level */
+    int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense FP32
+                                         inverse built on the matrix cores, rebuilt every N hierarchy
+                                         builds (0: red-black sweeps on the coarsest level) */
     double  t_fluid_init;             /* fluid t_temp fill, 293 (fluid_solver.h:108) */
     double  t_solid_init;             /* solid t_temp fill, T1 = 333 (mobs_solver.h:55) */
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
-    int32_t reserved[7];
+    int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
+    int32_t reserved[6];
 } hs_params;
 
 typedef struct {
