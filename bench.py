@@ -12,11 +12,11 @@ Each timed step is one full SIMPLE outer iteration (x/y/z momentum, p' assembly 
 solve, velocity / pressure correction, conjugate energy sweep, set-for-next), all
 inputs resident in HBM.
 
-roofline: the dominant kernel (one red-black colour pass of the finest multigrid
-level), timed live with HIP events on the library's stream around every launch in
-the timed region.  Algorithmic bytes per launch = 22 B per cell in FP32 (44 in FP64):
-the three face-coupling arrays and x read over all cells (16 B), ground and right-hand
-side read on the pass's colour (4 B), x written on that colour (2 B).
+roofline: the dominant kernel (one fused red-black Gauss-Seidel sweep of the finest
+multigrid level, both colours, out of place), timed live with HIP events on the
+library's stream around every launch in the timed region.  Algorithmic bytes per
+launch = 28 B per cell in FP32: the three face-coupling arrays, ground, right-hand side
+and the old iterate read (24 B), the new iterate written (4 B).
 
 cpu_baseline: the CPU oracle (reference algorithm: lexicographic Gauss-Seidel sweeps,
 unpreconditioned CG with HYPRE's stop rule), one core, first SIMPLE iteration of the
@@ -81,7 +81,7 @@ def pmc_traffic(n):
     (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a plain
     512 MiB device copy).  Counters cannot be read inside a timed run, so the figure is the
     profiled one; None for other grid sizes."""
-    f = ROOT / "profiles" / "k_mg_smooth_float_c3_256.md"
+    f = ROOT / "profiles" / "k_mg_rbsweep_float_c3_256.md"
     if n != 256 or not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -177,11 +177,11 @@ def main() -> int:
 
     n_launch, ms_total = prof
     avg_ms = ms_total / max(n_launch, 1)
-    bytes_per_launch = 22.0 * ncell
+    bytes_per_launch = 28.0 * ncell
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.n),
-                "kernel": "k_mg_smooth<float> (finest level, one red-black colour pass)",
+                "kernel": "k_mg_rbsweep<float> (finest level, one fused red-black sweep)",
                 "launches": n_launch, "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": bytes_per_launch}
 

@@ -106,6 +106,7 @@ struct MGLevel {
     long sy, sz, n;
     bool f32;
     void *cw, *cs, *cb, *g, *x, *b;
+    void* x2;          // second iterate buffer of the out-of-place fused red-black sweep
 };
 
 // Control block of the energy-equation BiCGStab (device resident, see hs_energy.hip).

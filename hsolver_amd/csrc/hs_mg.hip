@@ -135,6 +135,90 @@ __global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, i
     }
 }
 
+// One full red-black Gauss-Seidel sweep, out of place (xo -> xn), in one pass over the
+// level.  The first colour c1 is updated from xo, the second colour from the new first
+// colour values: exactly the two in-place colour passes (same operations, same order).
+// A workgroup owns a 64 x TY column tile and marches z through its chunk; the first
+// colour is recomputed on a one-cell ring around the tile (from xo, so tiles never read
+// each other's output) and kept for planes k-1, k, k+1 in LDS, where the second colour's
+// update of plane k reads it.  Taller tiles re-read fewer ring rows.
+template <typename T>
+__device__ __forceinline__ T relax_from(const Lv<T>& L, const T* x, long c) {
+    const T D = diag_of(L, c);
+    if (!(D > T(0))) return x[c];
+    const T off = L.cw[c] * x[c - 1] + L.cw[c + 1] * x[c + 1] + L.cs[c] * x[c - L.sy] + L.cs[c + L.sy] * x[c + L.sy] +
+                  L.cb[c] * x[c - L.sz] + L.cb[c + L.sz] * x[c + L.sz];
+    return (L.b[c] + off) / D;
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(BX * TY) void k_mg_rbsweep(Lv<T> L, const T* xo, T* xn, int c1, int kc, int gx, int gy,
+                                                        int gz, const SolveCtl* ctl) {
+    constexpr int RX = BX + 2, RY = TY + 2, NT = BX * TY;
+    if (ctl->done) return;
+    __shared__ T X1[3][RY * RX];
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i0 = 1 + tl.bx * BX, j0 = 1 + tl.by * TY;
+    const int k0 = 1 + tl.bz * kc, k1 = min(L.nz, k0 + kc - 1);
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    // first-colour values of plane k on the ring-1 region (0 outside the level)
+    auto first_plane = [&](int k) {
+        T* P = X1[k % 3];
+        for (int e = tid; e < RX * RY; e += NT) {
+            const int i = i0 - 1 + e % RX, j = j0 - 1 + e / RX;
+            T v = T(0);
+            if (k >= 1 && k <= L.nz && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && ((i + j + k) & 1) == c1)
+                v = relax_from(L, xo, i + L.sy * j + L.sz * k);
+            P[e] = v;
+        }
+    };
+    first_plane(k0 - 1);
+    first_plane(k0);
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool inside = i <= L.nx && j <= L.ny;
+    const int r = (threadIdx.y + 1) * RX + threadIdx.x + 1;
+    for (int k = k0; k <= k1; ++k) {
+        first_plane(k + 1);
+        __syncthreads();
+        if (inside) {
+            const long c = i + L.sy * j + L.sz * k;
+            const T* P0 = X1[k % 3];
+            T out;
+            if (((i + j + k) & 1) == c1) {
+                out = P0[r];
+            } else {
+                const T D = diag_of(L, c);
+                if (!(D > T(0))) {
+                    out = xo[c];
+                } else {
+                    const T* Pm = X1[(k + 2) % 3];
+                    const T* Pp = X1[(k + 1) % 3];
+                    const T off = L.cw[c] * P0[r - 1] + L.cw[c + 1] * P0[r + 1] + L.cs[c] * P0[r - RX] +
+                                  L.cs[c + L.sy] * P0[r + RX] + L.cb[c] * Pm[r] + L.cb[c + L.sz] * Pp[r];
+                    out = (L.b[c] + off) / D;
+                }
+            }
+            xn[c] = out;
+        }
+        __syncthreads();   // plane k-1's slot is rewritten by the next iteration
+    }
+}
+
+template <typename T, int TY>
+int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof) {
+    const int gx = (L.nx + BX - 1) / BX, gy = (L.ny + TY - 1) / TY;
+    // about 1M threads in flight: 4096 workgroups of 256 threads, fewer of larger tiles
+    long want = (4096L * 256 / (BX * TY)) / ((long)gx * gy);
+    if (want < 1) want = 1;
+    int kc = (int)((L.nz + want - 1) / want);
+    if (kc < 1) kc = 1;
+    const int gz = (L.nz + kc - 1) / kc;
+    const int slot = prof ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
+    k_mg_rbsweep<T, TY><<<dim3(gx * gy * gz), dim3(BX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
+    HS_TRY(prof_end(c, slot));
+    return HS_OK;
+}
+
 // b_c = sum over active children of (b - A x); x_c = 0
 template <typename T>
 __global__ void k_mg_restrict(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
@@ -342,7 +426,7 @@ __global__ __launch_bounds__(FLAT) void k_fcg_pupdate(long n, const T* z, double
 // ------------------------------------------------------------ host side ---
 template <typename T>
 int alloc_level(MGLevel* m) {
-    void** arrs[6] = {&m->cw, &m->cs, &m->cb, &m->g, &m->x, &m->b};
+    void** arrs[7] = {&m->cw, &m->cs, &m->cb, &m->g, &m->x, &m->b, &m->x2};
     for (void** a : arrs) {
         if (hipMalloc(a, m->n * sizeof(T)) != hipSuccess) return HS_E_NOMEM;
         if (hipMemset(*a, 0, m->n * sizeof(T)) != hipSuccess) return HS_E_HIP;
@@ -356,6 +440,19 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
     const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
     const int kc = hsd::kchunk(L.nz, tiles);
     const dim3 gr = hsd::grid3(L.nx, L.ny, L.nz, kc);
+    // pairs of sweeps as fused out-of-place passes x -> x2 -> x (the iterate ends in x)
+    const int c1 = forward ? 0 : 1;
+    const int rows = c->prm.mg_sweep_rows;
+    for (; sweeps >= 2; sweeps -= 2) {
+        for (int h = 0; h < 2; ++h) {
+            const T* src = (T*)(h == 0 ? m->x : m->x2);
+            T* dst = (T*)(h == 0 ? m->x2 : m->x);
+            const bool prof = m->level == 0;
+            if (rows == 4) HS_TRY((launch_rbsweep<T, 4>(c, L, src, dst, c1, prof)));
+            else if (rows == 16) HS_TRY((launch_rbsweep<T, 16>(c, L, src, dst, c1, prof)));
+            else HS_TRY((launch_rbsweep<T, 8>(c, L, src, dst, c1, prof)));
+        }
+    }
     for (int s = 0; s < sweeps; ++s)
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
@@ -426,7 +523,7 @@ int build_hierarchy(hs_ctx* c) {
             m->nx = nx; m->ny = ny; m->nz = nz;
             m->sy = nx + 2; m->sz = (long)(nx + 2) * (ny + 2); m->n = m->sz * (nz + 2);
             m->f32 = sizeof(T) == 4;
-            m->cw = m->cs = m->cb = m->g = m->x = m->b = nullptr;
+            m->cw = m->cs = m->cb = m->g = m->x = m->b = m->x2 = nullptr;
             c->mg.push_back(m);
             HS_TRY(alloc_level<T>(m));
             if (std::min(nx, std::min(ny, nz)) <= 4) break;
@@ -516,7 +613,7 @@ int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
 void gmg_free(hs_ctx* c) {
     dense_free(c);
     for (MGLevel* m : c->mg) {
-        void* arrs[6] = {m->cw, m->cs, m->cb, m->g, m->x, m->b};
+        void* arrs[7] = {m->cw, m->cs, m->cb, m->g, m->x, m->b, m->x2};
         for (void* a : arrs) if (a) hipFree(a);
         delete m;
     }

@@ -104,7 +104,8 @@ typedef struct {
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
     int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
-    int32_t reserved[6];
+    int32_t mg_sweep_rows;            /* rows (y) per tile of the fused red-black sweep: 4, 8 or 16 */
+    int32_t reserved[5];
 } hs_params;
 
 typedef struct {
@@ -247,7 +248,7 @@ int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem);
 
 /* ---- live kernel timing (no counterpart in the reference; its only timer is clock()
  *      around outer iterations, solver.h:53-54,156-158) ---- */
-#define HS_K_MG_SMOOTH0  1   /* one red-black colour pass on the finest multigrid level */
+#define HS_K_MG_SMOOTH0  1   /* one red-black Gauss-Seidel sweep (fused colours) on the finest multigrid level */
 #define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
 #define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
 #define HS_K_E_SMOOTH0   4   /* one red-black colour pass on the finest energy multigrid level */
