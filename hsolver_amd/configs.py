@@ -7,7 +7,8 @@ Configurations (synthetic inputs of the shapes the benchmark names; seeds fixed)
       (seed 2), Re = 400
   c3  256^3 unit box, 27 seeded solid blocks, k ratio 1000, lid + inlet, Re = 1000
   c4  128^3 on the c2 geometry scaled to a unit cube (adjoint case)
-  c5  512x256x256 channel, 64 seeded fins, tanh wall-stretched y/z grid
+  c5  512x256x256 channel, 64 seeded plate fins (20% of the cells), tanh wall-stretched
+      y/z grid with cell aspect ratio up to 50, Re 2000, k ratio 1000
 Reference-faithful inputs come from the .in files through hsolver_amd.mesh.
 
 Random numbers use splitmix64 -> (x >> 11) * 2^-53 so every platform draws the same
@@ -175,28 +176,64 @@ def c4(n: int = 128) -> Case:
                 f"unit cube {n}^3, centred heated cube, Re=400")
 
 
-def c5(nx=512, ny=256, nz=256, nfins=64, seed=5, beta=2.8) -> Case:
-    """512x256x256 channel [0,2]x[0,1]^2, y/z tanh-stretched, 64 heated fins (20% solid)."""
+def max_cell_aspect(xv, yv, zv) -> float:
+    """Largest ratio of a cell's longest to shortest side: a cell takes one width per
+    axis, so the extreme is the widest width of one axis over the narrowest of another."""
+    d = [np.diff(v) for v in (xv, yv, zv)]
+    return float(max(d[a].max() / d[b].min() for a in range(3) for b in range(3) if a != b))
+
+
+def c5_grid(nx=512, ny=256, nz=256, aspect=50.0):
+    """[0,2] x [0,1]^2 with y and z tanh-clustered to both walls; the clustering factor
+    is the one (bisection) whose largest cell aspect ratio equals `aspect`."""
     xv = uniform_axis(nx, 2.0)
-    yv = tanh_axis(ny, 1.0, beta)
-    zv = tanh_axis(nz, 1.0, beta)
-    r = splitmix64(seed, nfins * 3)
-    boxes = []
-    # plate fins standing on the y=0 wall, spread over x and z
-    fin_h = int(0.6 * ny)
-    area_each = 0.20 * nx * ny * nz / nfins / fin_h   # cells in x*z per fin
-    for f in range(nfins):
-        tz = max(1, int(round(area_each / max(1, int(0.8 * nx / 8)))))
-        tx = max(1, int(round(area_each / tz)))
-        tx = min(tx, nx // 8)
-        tz = max(1, int(round(area_each / tx)))
-        i0 = 2 + int(r[3 * f] * (nx - tx - 3))
-        k0 = 2 + int(r[3 * f + 1] * (nz - tz - 3))
-        boxes.append((i0, i0 + tx - 1, 1, fin_h, k0, k0 + tz - 1))
+    lo, hi = 0.1, 6.0
+    for _ in range(60):
+        beta = 0.5 * (lo + hi)
+        yv, zv = tanh_axis(ny, 1.0, beta), tanh_axis(nz, 1.0, beta)
+        if max_cell_aspect(xv, yv, zv) > aspect:
+            hi = beta
+        else:
+            lo = beta
+    return xv, tanh_axis(ny, 1.0, lo), tanh_axis(nz, 1.0, lo), lo
+
+
+def c5_fins(nx, ny, nz, nfins=64, seed=5, target_fraction=0.20):
+    """Plate fins standing on the y = 0 wall: height 0.6 of the channel, seeded x / z
+    positions, thickness nx/64 cells in x; the z extent grows (5% steps) until the fins
+    cover `target_fraction` of the cells.  Inlet and outlet columns stay fluid.
+    Returns (boxes, fraction)."""
+    r = splitmix64(seed, nfins * 2)
+    fin_h = int(round(0.6 * ny))
+    tx = max(1, nx // 64)
+    tz = max(1, int(target_fraction * nx * nz / (nfins * tx)))
+    for _ in range(80):
+        boxes = []
+        for f in range(nfins):
+            i0 = 2 + int(r[2 * f] * (nx - tx - 3))
+            k0 = 1 + int(r[2 * f + 1] * (nz - tz))
+            boxes.append((i0, i0 + tx - 1, 1, fin_h, k0, min(nz, k0 + tz - 1)))
+        # every fin spans y = 1..fin_h, so the solid fraction is that of the x-z footprint
+        foot = np.zeros((nz, nx), dtype=bool)
+        for (a0, a1, _, _, b0, b1) in boxes:
+            foot[b0 - 1:b1, a0 - 1:a1] = True
+        frac = float(foot.mean()) * fin_h / ny
+        if frac >= target_fraction:
+            break
+        tz = min(nz, int(tz * 1.05) + 1)
+    return boxes, frac
+
+
+def c5(nx=512, ny=256, nz=256, nfins=64, seed=5, aspect=50.0) -> Case:
+    """512x256x256 channel, wall-stretched y/z grid (cell aspect ratio up to 50), 64
+    seeded heated plate fins on the floor (20% of the cells solid), Re 2000, k ratio 1000."""
+    xv, yv, zv, beta = c5_grid(nx, ny, nz, aspect)
+    boxes, _ = c5_fins(nx, ny, nz, nfins, seed)
     ct = box_cells(nx, ny, nz, boxes)
-    p = config_params(re=2000.0, k_ratio=1000.0)
+    p = config_params(re=2000.0, k_ratio=1000.0, length=1.0)
+    p.t_wall = 0.0                       # temperatures as rises above the inlet / wall value
     q = np.where(ct.reshape(-1) == 1, 1.0, 0.0)
-    return Case("c5", xv, yv, zv, ct, p, q, f"channel {nx}x{ny}x{nz}, {nfins} fins, Re=2000")
+    return Case("c5", xv, yv, zv, ct, p, q, f"channel {nx}x{ny}x{nz}, {nfins} fins, Re=2000, aspect<={aspect:g}")
 
 
 def mode_r_params() -> abi.Params:

@@ -4,6 +4,8 @@
     iterations with the reference's solver settings (lexicographic sweeps,
     unpreconditioned CG at 1e-8), GPU vs the oracle's CPU path; every field within
     1e-6 relative L2, mass and energy balance residuals reported;
+  * the c5 geometry (stretched grid, plate fins) at 64x32x32: SIMPLE, energy solve and
+    adjoint against the oracle;
   * the same 200 iterations with the fast path (red-black sweeps, multigrid-preconditioned
     FCG, FP64 or mixed V-cycle) against the oracle's red-black sweeps with its
     unpreconditioned CG: the two Krylov methods meet the same 1e-8 stop rule, so the
@@ -70,3 +72,35 @@ def test_c1_fast_path_200_iterations(hs_lib, c1_oracle_rb, precision):
     errs = {k: rel_l2(fg[k], fo[k]) for k in fg}
     print(f"[c1] 200 iterations, RB + GMG ({precision}): rel-L2 vs oracle RB + CG {errs}", flush=True)
     assert all(e < 1e-6 for e in errs.values()), errs
+
+
+def test_c5_geometry_reduced(hs_lib, oracle_lib):
+    """The c5 geometry at 64x32x32 (tanh-stretched y/z at cell aspect ratio 50, plate fins):
+    10 SIMPLE iterations of the mixed fast path against the oracle's red-black + CG run,
+    then the energy solve and the fins' p-norm adjoint against the oracle's FP64 ones."""
+    from helpers import cell_volumes, solid_pnorm
+    case = configs.c5(64, 32, 32, nfins=8)
+    case.params.order = abi.HS_ORDER_RB
+    case.params.p_tol = 1e-10
+    gp = abi.Params.from_buffer_copy(case.params)
+    gp.precond, gp.precision, gp.mg_dense_coarse = abi.HS_PRECOND_GMG, abi.HS_PREC_MIXED, 1
+    g, o = gpu_ctx(hs_lib, case, gp), oracle_ctx(oracle_lib, case)
+    g.init(); o.init()
+    for it in range(10):
+        sg = g.step(); o.step()
+        assert sg["p_true_relres"] < 1e-9, (it, sg)
+    fg, fo = g.fields(), o.fields()
+    errs = {k: rel_l2(fg[k], fo[k]) for k in fg}
+    print(f"[c5 reduced] 10 iterations, rel-L2 vs oracle {errs}", flush=True)
+    assert all(e < 1e-6 for e in errs.values()), errs
+    o.load_state(g.state())
+    ig = g.energy_solve(1e-12, 3000)
+    io = o.energy_solve(1e-12, 20000)
+    assert ig["converged"] == 1 and io["converged"] == 1, (ig, io)
+    assert rel_l2(g.cells("tf"), o.cells("tf")) < 1e-9
+    nx, ny, nz = case.shape
+    solid = case.grid.cell_type.reshape(nz, ny, nx) == 1
+    _, dJdT = solid_pnorm(o.cells("tf"), solid, cell_volumes(case))
+    dq_g, dk_g, _ = g.adjoint_energy(dJdT, 1e-12, 3000)
+    dq_o, dk_o, _ = o.adjoint_energy(dJdT, 1e-12, 20000)
+    assert rel_l2(dq_g, dq_o) < 1e-8 and rel_l2(dk_g, dk_o) < 1e-7
