@@ -274,7 +274,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_coarse_sweeps = 40;
     p->mg_dense_coarse = 0;
     p->mg_dense_refine = 2;
-    p->mg_sweep_rows = 8;
+    p->mg_sweep_rows = 0;
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
     p->t_wall = 293.0;

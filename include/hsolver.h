@@ -104,7 +104,8 @@ typedef struct {
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
     int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
-    int32_t mg_sweep_rows;            /* rows (y) per tile of the fused red-black sweep: 4, 8 or 16 */
+    int32_t mg_sweep_rows;            /* rows (y) per tile of the fused red-black sweep: 4, 8 or 16;
+                                         0 = 8 for the float V-cycle, 16 for the double one */
     int32_t reserved[5];
 } hs_params;
 
