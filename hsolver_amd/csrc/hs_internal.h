@@ -107,6 +107,7 @@ struct MGLevel {
     bool f32;
     void *cw, *cs, *cb, *g, *x, *b;
     void* x2;          // second iterate buffer of the out-of-place fused red-black sweep
+    uint8_t* act;      // 1 where the level operator has a row (D > 0), set at every build
 };
 
 // Control block of the energy-equation BiCGStab (device resident, see hs_energy.hip).

@@ -219,26 +219,44 @@ int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool
     return HS_OK;
 }
 
-// b_c = sum over active children of (b - A x); x_c = 0
+// b_c = sum over active children of (b - A x); x_c = 0.
+// One thread per fine column of a 64 x 4 fine tile (coarse 32 x 2): each thread forms
+// the residuals of its two fine cells of coarse plane K, and the 2 x 2 in-plane sums go
+// through LDS.
 template <typename T>
-__global__ void k_mg_restrict(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
+__global__ __launch_bounds__(BT) void k_mg_restrict(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz,
+                                                   const SolveCtl* ctl) {
     if (ctl->done) return;
-    const int I = 1 + blockIdx.x * blockDim.x + threadIdx.x, J = 1 + blockIdx.y, K = 1 + blockIdx.z;
-    if (I > C.nx) return;
-    const int i0 = 2 * I - 1, j0 = 2 * J - 1, k0 = 2 * K - 1;
-    const int ni = (2 * I <= F.nx) ? 2 : 1, nj = (2 * J <= F.ny) ? 2 : 1, nk = (2 * K <= F.nz) ? 2 : 1;
-    T r = 0;
-    for (int dk = 0; dk < nk; ++dk)
-        for (int dj = 0; dj < nj; ++dj)
-            for (int di = 0; di < ni; ++di) {
-                const long f = (i0 + di) + F.sy * (j0 + dj) + F.sz * (k0 + dk);
+    __shared__ T red[BY][BX];
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    const bool inside = i <= F.nx && j <= F.ny;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    const int ci = t % (BX / 2), cj = t / (BX / 2);
+    const int I = 1 + tl.bx * (BX / 2) + ci, J = 1 + tl.by * (BY / 2) + cj;
+    const bool owner = t < (BX / 2) * (BY / 2) && I <= C.nx && J <= C.ny;
+    const int K0 = 1 + tl.bz * kc, K1 = min(C.nz, K0 + kc - 1);
+    for (int K = K0; K <= K1; ++K) {
+        T r = 0;
+        if (inside) {
+            for (int dk = 0; dk < 2; ++dk) {
+                const int k = 2 * K - 1 + dk;
+                if (k > F.nz) break;
+                const long f = i + F.sy * j + F.sz * k;
                 const T D = diag_of(F, f);
-                if (!(D > T(0))) continue;
-                r += F.b[f] - (D * F.x[f] - offsum(F, f));
+                if (D > T(0)) r += F.b[f] - (D * F.x[f] - offsum(F, f));
             }
-    const long cc = I + C.sy * J + C.sz * K;
-    C.b[cc] = r;
-    C.x[cc] = T(0);
+        }
+        red[threadIdx.y][threadIdx.x] = r;
+        __syncthreads();
+        if (owner) {
+            const T s = (red[2 * cj][2 * ci] + red[2 * cj][2 * ci + 1]) + (red[2 * cj + 1][2 * ci] + red[2 * cj + 1][2 * ci + 1]);
+            const long cc = I + C.sy * J + C.sz * K;
+            C.b[cc] = s;
+            C.x[cc] = T(0);
+        }
+        __syncthreads();
+    }
 }
 
 // Trilinear interpolation of the coarse correction to fine cell (i,j,k) (1-based): the
@@ -260,15 +278,61 @@ __device__ __forceinline__ T prolong_value(const Lv<T>& C, int i, int j, int k) 
     return a * zlo + b * zhi;
 }
 
-// x_f += P x_c on active fine cells
+// x_f += P x_c on active fine cells (64 x 4 fine columns per workgroup, z chunks)
 template <typename T>
-__global__ void k_mg_prolong(Lv<T> F, Lv<T> C, const SolveCtl* ctl) {
+__global__ __launch_bounds__(BT) void k_mg_prolong(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz,
+                                                  const SolveCtl* ctl) {
     if (ctl->done) return;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    if (i > F.nx || j > F.ny) return;
+    const int k0 = 1 + tl.bz * kc, k1 = min(F.nz, k0 + kc - 1);
+    for (int k = k0; k <= k1; ++k) {
+        const long f = i + F.sy * j + F.sz * k;
+        if (F.act[f]) F.x[f] += prolong_value(C, i, j, k);
+    }
+}
+
+// launch shapes of the tiled transfer kernels over a level of nx x ny x nz cells
+struct TileShape {
+    int kc;
+    dim3 gr;
+};
+inline TileShape tile_shape(int nx, int ny, int nz) {
+    const long tiles = (long)((nx + BX - 1) / BX) * ((ny + BY - 1) / BY);
+    TileShape s;
+    s.kc = hsd::kchunk(nz, tiles);
+    s.gr = hsd::grid3(nx, ny, nz, s.kc);
+    return s;
+}
+
+template <typename T>
+int launch_restrict(hs_ctx* c, const Lv<T>& F, const Lv<T>& C) {
+    // fine-cell tiles in x and y, coarse planes in z
+    const long tiles = (long)((F.nx + BX - 1) / BX) * ((F.ny + BY - 1) / BY);
+    TileShape s;
+    s.kc = hsd::kchunk(C.nz, tiles);
+    s.gr = dim3((F.nx + BX - 1) / BX, (F.ny + BY - 1) / BY, (C.nz + s.kc - 1) / s.kc);
+    k_mg_restrict<T><<<hsd::grid1(s.gr), dim3(BX, BY), 0, c->stream>>>(F, C, s.kc, s.gr.x, s.gr.y, s.gr.z, c->sctl);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+template <typename T>
+int launch_prolong(hs_ctx* c, const Lv<T>& F, const Lv<T>& C) {
+    const TileShape s = tile_shape(F.nx, F.ny, F.nz);
+    k_mg_prolong<T><<<hsd::grid1(s.gr), dim3(BX, BY), 0, c->stream>>>(F, C, s.kc, s.gr.x, s.gr.y, s.gr.z, c->sctl);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
+// active-row mask of a level (one byte per cell instead of seven coefficient reads)
+template <typename T>
+__global__ void k_mg_mask(Lv<T> L, uint8_t* act) {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
-    if (i > F.nx) return;
-    const long f = i + F.sy * j + F.sz * k;
-    if (!(diag_of(F, f) > T(0))) return;
-    F.x[f] += prolong_value(C, i, j, k);
+    if (i > L.nx) return;
+    const long c = i + L.sy * j + L.sz * k;
+    act[c] = diag_of(L, c) > T(0) ? 1 : 0;
 }
 
 // ---- single-workgroup tail: every level from `t` down, in one launch ----
@@ -327,7 +391,7 @@ __device__ void tail_prolong(const Lv<T>& F, const Lv<T>& C) {
     for (long t = threadIdx.x; t < nf; t += TAIL_THREADS) {
         const int i = 1 + (int)(t % F.nx), j = 1 + (int)((t / F.nx) % F.ny), k = 1 + (int)(t / ((long)F.nx * F.ny));
         const long f = i + F.sy * j + F.sz * k;
-        if (!(diag_of(F, f) > T(0))) continue;
+        if (!F.act[f]) continue;
         F.x[f] += prolong_value(C, i, j, k);
     }
     __syncthreads();
@@ -431,6 +495,8 @@ int alloc_level(MGLevel* m) {
         if (hipMalloc(a, m->n * sizeof(T)) != hipSuccess) return HS_E_NOMEM;
         if (hipMemset(*a, 0, m->n * sizeof(T)) != hipSuccess) return HS_E_HIP;
     }
+    if (hipMalloc(&m->act, m->n) != hipSuccess) return HS_E_NOMEM;
+    if (hipMemset(m->act, 0, m->n) != hipSuccess) return HS_E_HIP;
     return HS_OK;
 }
 
@@ -476,14 +542,12 @@ int vcycle(hs_ctx* c) {
         for (int l = 0; l + 1 < nl; ++l) {
             HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
             Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-            k_mg_restrict<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, c->sctl);
-            HS_CHECK(hipGetLastError());
+            HS_TRY(launch_restrict<T>(c, F, C));
         }
         HS_TRY(dense_solve(c, c->mg[nl - 1], sizeof(T) == sizeof(float)));
         for (int l = nl - 2; l >= 0; --l) {
             Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-            k_mg_prolong<T><<<dim3((F.nx + 63) / 64, F.ny, F.nz), 64, 0, c->stream>>>(F, C, c->sctl);
-            HS_CHECK(hipGetLastError());
+            HS_TRY(launch_prolong<T>(c, F, C));
             HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
         }
         return HS_OK;
@@ -495,8 +559,7 @@ int vcycle(hs_ctx* c) {
     for (int l = 0; l < t; ++l) {
         HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-        k_mg_restrict<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, c->sctl);
-        HS_CHECK(hipGetLastError());
+        HS_TRY(launch_restrict<T>(c, F, C));
     }
     LvPack<T> P;
     for (int l = 0; l < nl && l < MAX_LEVELS; ++l) P.l[l] = lv<T>(c->mg[l]);
@@ -505,8 +568,7 @@ int vcycle(hs_ctx* c) {
     HS_CHECK(hipGetLastError());
     for (int l = t - 1; l >= 0; --l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-        k_mg_prolong<T><<<dim3((F.nx + 63) / 64, F.ny, F.nz), 64, 0, c->stream>>>(F, C, c->sctl);
-        HS_CHECK(hipGetLastError());
+        HS_TRY(launch_prolong<T>(c, F, C));
         HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
     }
     return HS_OK;
@@ -525,6 +587,7 @@ int build_hierarchy(hs_ctx* c) {
             m->sy = nx + 2; m->sz = (long)(nx + 2) * (ny + 2); m->n = m->sz * (nz + 2);
             m->f32 = sizeof(T) == 4;
             m->cw = m->cs = m->cb = m->g = m->x = m->b = m->x2 = nullptr;
+            m->act = nullptr;
             c->mg.push_back(m);
             HS_TRY(alloc_level<T>(m));
             if (std::min(nx, std::min(ny, nz)) <= 4) break;
@@ -544,6 +607,10 @@ int build_hierarchy(hs_ctx* c) {
         const double pw = std::ldexp(1.0, (int)l);
         const T gscale = (T)((pw + 1.0) / (2.0 * pw + 1.0));
         k_mg_coarsen<T><<<dim3((C.nx + 63) / 64, C.ny, C.nz), 64, 0, c->stream>>>(F, C, gscale);
+        HS_CHECK(hipGetLastError());
+    }
+    for (MGLevel* m : c->mg) {
+        k_mg_mask<T><<<dim3((m->nx + 63) / 64, m->ny, m->nz), 64, 0, c->stream>>>(lv<T>(m), m->act);
         HS_CHECK(hipGetLastError());
     }
     if (c->prm.mg_dense_coarse > 0 && (long)c->mg.back()->nx * c->mg.back()->ny * c->mg.back()->nz <= dense_max_cells())
@@ -614,7 +681,7 @@ int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
 void gmg_free(hs_ctx* c) {
     dense_free(c);
     for (MGLevel* m : c->mg) {
-        void* arrs[7] = {m->cw, m->cs, m->cb, m->g, m->x, m->b, m->x2};
+        void* arrs[8] = {m->cw, m->cs, m->cb, m->g, m->x, m->b, m->x2, m->act};
         for (void* a : arrs) if (a) hipFree(a);
         delete m;
     }

@@ -15,11 +15,13 @@ struct Lv {
     int nx, ny, nz;
     long sy, sz;
     T *cw, *cs, *cb, *g, *x, *b;
+    const uint8_t* act;   // active-row mask (D > 0)
 };
 
 template <typename T>
 inline Lv<T> lv(const MGLevel* m) {
-    return Lv<T>{m->nx, m->ny, m->nz, m->sy, m->sz, (T*)m->cw, (T*)m->cs, (T*)m->cb, (T*)m->g, (T*)m->x, (T*)m->b};
+    return Lv<T>{m->nx, m->ny, m->nz, m->sy, m->sz, (T*)m->cw, (T*)m->cs, (T*)m->cb, (T*)m->g, (T*)m->x, (T*)m->b,
+                 m->act};
 }
 
 template <typename T>
