@@ -45,7 +45,8 @@ class Params(C.Structure):
                 ("mg_coarse_sweeps", C.c_int32), ("mg_dense_coarse", C.c_int32),
                 ("t_fluid_init", C.c_double), ("t_solid_init", C.c_double), ("t_wall", C.c_double),
                 ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
-                ("mg_sweep_rows", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -110,6 +111,8 @@ def params_default(mode: int) -> Params:
     p.mg_dense_coarse = 0
     p.mg_dense_refine = 2
     p.mg_sweep_rows = 0
+    p.mg_pre0 = -1
+    p.mg_post0 = 1
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0
     p.t_wall = 293.0

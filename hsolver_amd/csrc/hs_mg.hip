@@ -506,18 +506,21 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
     const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
     const int kc = hsd::kchunk(L.nz, tiles);
     const dim3 gr = hsd::grid3(L.nx, L.ny, L.nz, kc);
-    // pairs of sweeps as fused out-of-place passes x -> x2 -> x (the iterate ends in x)
+    // fused out-of-place sweeps x -> x2; after each the level's two iterate buffers swap,
+    // so m->x always holds the latest iterate (every user reads it at launch time)
     const int c1 = forward ? 0 : 1;
     // tile rows of the fused sweep: 0 picks the measured best (8 in float, 16 in double)
     const int rows = c->prm.mg_sweep_rows > 0 ? c->prm.mg_sweep_rows : (sizeof(T) == sizeof(float) ? 8 : 16);
-    for (; sweeps >= 2; sweeps -= 2) {
-        for (int h = 0; h < 2; ++h) {
-            const T* src = (T*)(h == 0 ? m->x : m->x2);
-            T* dst = (T*)(h == 0 ? m->x2 : m->x);
+    if (sweeps > 0 && rows > 0) {
+        for (; sweeps > 0; --sweeps) {
+            const T* src = (T*)m->x;
+            T* dst = (T*)m->x2;
             const bool prof = m->level == 0;
             if (rows == 4) HS_TRY((launch_rbsweep<T, 4>(c, L, src, dst, c1, prof)));
             else if (rows == 16) HS_TRY((launch_rbsweep<T, 16>(c, L, src, dst, c1, prof)));
             else HS_TRY((launch_rbsweep<T, 8>(c, L, src, dst, c1, prof)));
+            std::swap(m->x, m->x2);
+            L.x = (T*)m->x;
         }
     }
     for (int s = 0; s < sweeps; ++s)
@@ -531,6 +534,12 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
     return HS_OK;
 }
 
+// sweeps before / after the coarse correction on level l (the finest level may differ)
+inline int pre_sweeps(const hs_ctx* c, int l) { return (l == 0 && c->prm.mg_pre0 >= 0) ? c->prm.mg_pre0 : c->prm.mg_pre; }
+inline int post_sweeps(const hs_ctx* c, int l) {
+    return (l == 0 && c->prm.mg_post0 >= 0) ? c->prm.mg_post0 : c->prm.mg_post;
+}
+
 // One V-cycle on the level-0 arrays: b holds the residual, x starts at 0 and leaves
 // holding the preconditioned residual (both prepared / consumed by the FCG kernels).
 template <typename T>
@@ -540,7 +549,7 @@ int vcycle(hs_ctx* c) {
         // every level but the coarsest smoothed by the grid kernels, the coarsest solved
         // directly with its dense inverse (+ FP64 refinement)
         for (int l = 0; l + 1 < nl; ++l) {
-            HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
+            HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
             Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
             HS_TRY(launch_restrict<T>(c, F, C));
         }
@@ -548,7 +557,7 @@ int vcycle(hs_ctx* c) {
         for (int l = nl - 2; l >= 0; --l) {
             Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
             HS_TRY(launch_prolong<T>(c, F, C));
-            HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
+            HS_TRY(smooth<T>(c, c->mg[l], post_sweeps(c, l), false));
         }
         return HS_OK;
     }
@@ -557,7 +566,7 @@ int vcycle(hs_ctx* c) {
     while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
     if (t >= nl) t = nl - 1;
     for (int l = 0; l < t; ++l) {
-        HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_pre, true));
+        HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
         HS_TRY(launch_restrict<T>(c, F, C));
     }
@@ -569,7 +578,7 @@ int vcycle(hs_ctx* c) {
     for (int l = t - 1; l >= 0; --l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
         HS_TRY(launch_prolong<T>(c, F, C));
-        HS_TRY(smooth<T>(c, c->mg[l], c->prm.mg_post, false));
+        HS_TRY(smooth<T>(c, c->mg[l], post_sweeps(c, l), false));
     }
     return HS_OK;
 }
@@ -629,12 +638,14 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
     double *r = c->kr, *p = c->kp, *q = c->ks;
     T* b0 = (T*)c->mg[0]->b;
-    T* z = (T*)c->mg[0]->x;
+    // the level-0 iterate (the V-cycle's output z) moves between two buffers with the
+    // smoother's sweeps: read its current pointer at every use
+    auto z = [&]() { return (T*)c->mg[0]->x; };
     const int G = FLAT_BLOCKS;
-    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z, c->partials);
+    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), c->partials);
     HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));
     HS_TRY(vcycle<T>(c));
-    k_fcg_start<T><<<G, FLAT, 0, c->stream>>>(n, r, z, p, c->partials, c->sctl);
+    k_fcg_start<T><<<G, FLAT, 0, c->stream>>>(n, r, z(), p, c->partials, c->sctl);
     HS_TRY(launch_finish(c, FIN_FCG_START, G, nullptr));
     HS_CHECK(hipGetLastError());
     int launched = 0;
@@ -642,12 +653,12 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
             HS_TRY(cg_spmv_dot(c, p, q));
-            k_fcg_update<T><<<G, FLAT, 0, c->stream>>>(n, x, r, p, q, b0, z, c->partials, c->sctl);
+            k_fcg_update<T><<<G, FLAT, 0, c->stream>>>(n, x, r, p, q, b0, z(), c->partials, c->sctl);
             HS_TRY(launch_finish(c, FIN_FCG_UPDATE, G, nullptr));
             HS_TRY(vcycle<T>(c));
-            k_fcg_beta<T><<<G, FLAT, 0, c->stream>>>(n, z, r, q, c->partials, c->sctl);
+            k_fcg_beta<T><<<G, FLAT, 0, c->stream>>>(n, z(), r, q, c->partials, c->sctl);
             HS_TRY(launch_finish(c, FIN_FCG_BETA, G, nullptr));
-            k_fcg_pupdate<T><<<G, FLAT, 0, c->stream>>>(n, z, p, c->sctl);
+            k_fcg_pupdate<T><<<G, FLAT, 0, c->stream>>>(n, z(), p, c->sctl);
         }
         HS_CHECK(hipGetLastError());
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));

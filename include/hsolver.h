@@ -106,7 +106,9 @@ typedef struct {
     int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
     int32_t mg_sweep_rows;            /* rows (y) per tile of the fused red-black sweep: 4, 8 or 16;
                                          0 = 8 for the float V-cycle, 16 for the double one */
-    int32_t reserved[5];
+    int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post);
+                                         defaults -1 / 1, i.e. V(2,1) there */
+    int32_t reserved[3];
 } hs_params;
 
 typedef struct {
