@@ -263,13 +263,40 @@ __device__ __forceinline__ T e_residual_sum(const EL<T>& F, int I, int J, int K,
 }
 
 template <typename T>
-__global__ void k_e_restrict(EL<T> F, EL<T> C, int tr, const int* skip) {
+__global__ __launch_bounds__(BT) void k_e_restrict(EL<T> F, EL<T> C, int tr, int kc, int gx, int gy, int gz,
+                                                   const int* skip) {
+    // one thread per fine column of a 64 x 4 fine tile (coarse 32 x 2); the 2 x 2 in-plane
+    // sums of the children's residuals go through LDS
     if (*skip) return;
-    const int I = 1 + blockIdx.x * blockDim.x + threadIdx.x, J = 1 + blockIdx.y, K = 1 + blockIdx.z;
-    if (I > C.nx) return;
-    const long cc = I + C.sy * J + C.sz * K;
-    C.b[cc] = e_residual_sum(F, I, J, K, tr);
-    C.x[cc] = T(0);
+    __shared__ T red[BY][BX];
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    const bool inside = i <= F.nx && j <= F.ny;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    const int ci = t % (BX / 2), cj = t / (BX / 2);
+    const int I = 1 + tl.bx * (BX / 2) + ci, J = 1 + tl.by * (BY / 2) + cj;
+    const bool owner = t < (BX / 2) * (BY / 2) && I <= C.nx && J <= C.ny;
+    const int K0 = 1 + tl.bz * kc, K1 = min(C.nz, K0 + kc - 1);
+    for (int K = K0; K <= K1; ++K) {
+        T r = 0;
+        if (inside) {
+            for (int dk = 0; dk < 2; ++dk) {
+                const int k = 2 * K - 1 + dk;
+                if (k > F.nz) break;
+                const long f = i + F.sy * j + F.sz * k;
+                const Row<T> rr = lrow(F, f, tr);
+                r += F.b[f] - (rr.p * F.x[f] - offs<T, T>(rr, F.x, f, F.sy, F.sz));
+            }
+        }
+        red[threadIdx.y][threadIdx.x] = r;
+        __syncthreads();
+        if (owner) {
+            const long cc = I + C.sy * J + C.sz * K;
+            C.b[cc] = (red[2 * cj][2 * ci] + red[2 * cj][2 * ci + 1]) + (red[2 * cj + 1][2 * ci] + red[2 * cj + 1][2 * ci + 1]);
+            C.x[cc] = T(0);
+        }
+        __syncthreads();
+    }
 }
 
 // trilinear interpolation of the coarse iterate (27/64, 9/64, 3/64, 1/64 weights);
@@ -288,11 +315,13 @@ __device__ __forceinline__ T e_prolong_value(const EL<T>& C, int i, int j, int k
 }
 
 template <typename T>
-__global__ void k_e_prolong(EL<T> F, EL<T> C, const int* skip) {
+__global__ __launch_bounds__(BT) void k_e_prolong(EL<T> F, EL<T> C, int kc, int gx, int gy, int gz, const int* skip) {
     if (*skip) return;
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
-    if (i > F.nx) return;
-    F.x[i + F.sy * j + F.sz * k] += e_prolong_value(C, i, j, k);
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    if (i > F.nx || j > F.ny) return;
+    const int k0 = 1 + tl.bz * kc, k1 = min(F.nz, k0 + kc - 1);
+    for (int k = k0; k <= k1; ++k) F.x[i + F.sy * j + F.sz * k] += e_prolong_value(C, i, j, k);
 }
 
 // single-workgroup tail for the small levels (see the pressure hierarchy)
@@ -716,7 +745,12 @@ int e_vcycle(hs_ctx* c, int tr) {
     for (int l = 0; l < t; ++l) {
         HS_TRY(e_smooth<T>(c, E->lv[l], tr, c->prm.mg_pre, true, skip, l == 0));
         const ELevel* m = E->lv[l + 1];
-        k_e_restrict<T><<<cells_grid(m->nx, m->ny, m->nz), 64, 0, c->stream>>>(el<T>(E->lv[l]), el<T>(m), tr, skip);
+        const ELevel* f = E->lv[l];
+        const long tiles = (long)((f->nx + BX - 1) / BX) * ((f->ny + BY - 1) / BY);
+        const int kc = hsd::kchunk(m->nz, tiles);
+        const dim3 gr((f->nx + BX - 1) / BX, (f->ny + BY - 1) / BY, (m->nz + kc - 1) / kc);
+        k_e_restrict<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(el<T>(f), el<T>(m), tr, kc, gr.x, gr.y, gr.z,
+                                                                        skip);
         HS_CHECK(hipGetLastError());
     }
     ELPack<T> P;
@@ -726,7 +760,11 @@ int e_vcycle(hs_ctx* c, int tr) {
     HS_CHECK(hipGetLastError());
     for (int l = t - 1; l >= 0; --l) {
         const ELevel* f = E->lv[l];
-        k_e_prolong<T><<<cells_grid(f->nx, f->ny, f->nz), 64, 0, c->stream>>>(el<T>(f), el<T>(E->lv[l + 1]), skip);
+        const long tiles = (long)((f->nx + BX - 1) / BX) * ((f->ny + BY - 1) / BY);
+        const int kc = hsd::kchunk(f->nz, tiles);
+        const dim3 gr = hsd::grid3(f->nx, f->ny, f->nz, kc);
+        k_e_prolong<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(el<T>(f), el<T>(E->lv[l + 1]), kc, gr.x, gr.y,
+                                                                       gr.z, skip);
         HS_CHECK(hipGetLastError());
         HS_TRY(e_smooth<T>(c, f, tr, c->prm.mg_post, false, skip, l == 0));
     }
