@@ -71,3 +71,21 @@ def cell_volumes(case):
     g = case.grid
     dx, dy, dz = np.diff(g.xv), np.diff(g.yv), np.diff(g.zv)
     return dz[:, None, None] * dy[None, :, None] * dx[None, None, :]
+
+
+def export_p(ctx):
+    """The oracle's p' operator (fluid rows, initMap order) as scipy CSR + rhs + row cells."""
+    lib = ctx.lib
+    I64 = C.c_int64
+    lib.or_export_csr.argtypes = [C.c_void_p, C.POINTER(I64), C.POINTER(I64), C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]
+    nr, nz = I64(), I64()
+    assert lib.or_export_csr(ctx.h, C.byref(nr), C.byref(nz), None, None, None, None, None) == 0
+    rp = np.zeros(nr.value + 1, np.int64)
+    cols = np.zeros(nz.value, np.int64)
+    vals = np.zeros(nz.value)
+    rhs = np.zeros(nr.value)
+    cell = np.zeros(nr.value, np.int64)
+    assert lib.or_export_csr(ctx.h, C.byref(nr), C.byref(nz), rp.ctypes.data, cols.ctypes.data, vals.ctypes.data,
+                             rhs.ctypes.data, cell.ctypes.data) == 0
+    return sp.csr_matrix((vals, cols, rp), shape=(nr.value, nr.value)), rhs, cell
