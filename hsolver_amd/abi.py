@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("t_fluid_init", C.c_double), ("t_solid_init", C.c_double), ("t_wall", C.c_double),
                 ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("mg_cycle", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):
@@ -113,6 +113,7 @@ def params_default(mode: int) -> Params:
     p.mg_sweep_rows = 0
     p.mg_pre0 = -1
     p.mg_post0 = 1
+    p.mg_cycle = 1
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0
     p.t_wall = 293.0

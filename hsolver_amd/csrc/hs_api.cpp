@@ -277,6 +277,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_sweep_rows = 0;
     p->mg_pre0 = -1;
     p->mg_post0 = 1;     /* V(2,1) on the finest level: fastest measured at 256^3 (13 FCG iterations) */
+    p->mg_cycle = 1;
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
     p->t_wall = 293.0;

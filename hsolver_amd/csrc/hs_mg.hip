@@ -542,45 +542,43 @@ inline int post_sweeps(const hs_ctx* c, int l) {
 
 // One V-cycle on the level-0 arrays: b holds the residual, x starts at 0 and leaves
 // holding the preconditioned residual (both prepared / consumed by the FCG kernels).
+// Multigrid cycle from level l down to the level `lend` that is solved directly (the dense
+// inverse, or the single-workgroup tail kernel for every level from `lend` down).
+// Level 0 is visited once; each coarser level recurses mg_cycle times (1: V-cycle,
+// 2: W-cycle below the finest level).  The second visit starts from the first's
+// iterate: restriction forms the residual of the current iterate, the coarse correction
+// is added by prolongation.
+template <typename T>
+int cycle(hs_ctx* c, int l, int lend, bool dense) {
+    const int nl = (int)c->mg.size();
+    if (l == lend) {
+        if (dense) return dense_solve(c, c->mg[l], sizeof(T) == sizeof(float));
+        LvPack<T> P;
+        for (int q = 0; q < nl && q < MAX_LEVELS; ++q) P.l[q] = lv<T>(c->mg[q]);
+        k_mg_tail<T><<<1, TAIL_THREADS, 0, c->stream>>>(P, l, nl, c->prm.mg_pre, c->prm.mg_post,
+                                                        c->prm.mg_coarse_sweeps, c->sctl);
+        HS_CHECK(hipGetLastError());
+        return HS_OK;
+    }
+    HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
+    HS_TRY(launch_restrict<T>(c, lv<T>(c->mg[l]), lv<T>(c->mg[l + 1])));
+    const int reps = (l == 0 || l + 1 == lend) ? 1 : std::max(1, c->prm.mg_cycle);
+    for (int r = 0; r < reps; ++r) HS_TRY(cycle<T>(c, l + 1, lend, dense));
+    HS_TRY(launch_prolong<T>(c, lv<T>(c->mg[l]), lv<T>(c->mg[l + 1])));
+    return smooth<T>(c, c->mg[l], post_sweeps(c, l), false);
+}
+
+// One cycle on the level-0 arrays: b holds the residual, x starts at 0 and leaves holding
+// the preconditioned residual (both prepared / consumed by the FCG kernels).
 template <typename T>
 int vcycle(hs_ctx* c) {
     const int nl = (int)c->mg.size();
-    if (c->prm.mg_dense_coarse > 0 && c->dense) {
-        // every level but the coarsest smoothed by the grid kernels, the coarsest solved
-        // directly with its dense inverse (+ FP64 refinement)
-        for (int l = 0; l + 1 < nl; ++l) {
-            HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
-            Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-            HS_TRY(launch_restrict<T>(c, F, C));
-        }
-        HS_TRY(dense_solve(c, c->mg[nl - 1], sizeof(T) == sizeof(float)));
-        for (int l = nl - 2; l >= 0; --l) {
-            Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-            HS_TRY(launch_prolong<T>(c, F, C));
-            HS_TRY(smooth<T>(c, c->mg[l], post_sweeps(c, l), false));
-        }
-        return HS_OK;
-    }
+    if (c->prm.mg_dense_coarse > 0 && c->dense) return cycle<T>(c, 0, nl - 1, true);
     // first level small enough for the single-workgroup tail
     int t = 0;
     while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
     if (t >= nl) t = nl - 1;
-    for (int l = 0; l < t; ++l) {
-        HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
-        Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-        HS_TRY(launch_restrict<T>(c, F, C));
-    }
-    LvPack<T> P;
-    for (int l = 0; l < nl && l < MAX_LEVELS; ++l) P.l[l] = lv<T>(c->mg[l]);
-    k_mg_tail<T><<<1, TAIL_THREADS, 0, c->stream>>>(P, t, nl, c->prm.mg_pre, c->prm.mg_post,
-                                                    c->prm.mg_coarse_sweeps, c->sctl);
-    HS_CHECK(hipGetLastError());
-    for (int l = t - 1; l >= 0; --l) {
-        Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
-        HS_TRY(launch_prolong<T>(c, F, C));
-        HS_TRY(smooth<T>(c, c->mg[l], post_sweeps(c, l), false));
-    }
-    return HS_OK;
+    return cycle<T>(c, 0, t, false);
 }
 
 template <typename T>

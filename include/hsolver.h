@@ -108,7 +108,8 @@ typedef struct {
                                          0 = 8 for the float V-cycle, 16 for the double one */
     int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post);
                                          defaults -1 / 1, i.e. V(2,1) there */
-    int32_t reserved[3];
+    int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
+    int32_t reserved[2];
 } hs_params;
 
 typedef struct {
