@@ -137,84 +137,170 @@ __global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, i
 
 // One full red-black Gauss-Seidel sweep, out of place (xo -> xn), in one pass over the
 // level.  The first colour c1 is updated from xo, the second colour from the new first
-// colour values: exactly the two in-place colour passes (same operations, same order).
-// A workgroup owns a 64 x TY column tile and marches z through its chunk; the first
-// colour is recomputed on a one-cell ring around the tile (from xo, so tiles never read
-// each other's output) and kept for planes k-1, k, k+1 in LDS, where the second colour's
-// update of plane k reads it.  Taller tiles re-read fewer ring rows.
-template <typename T>
-__device__ __forceinline__ T relax_from(const Lv<T>& L, const T* x, long c) {
-    const T D = diag_of(L, c);
-    if (!(D > T(0))) return x[c];
-    const T off = L.cw[c] * x[c - 1] + L.cw[c + 1] * x[c + 1] + L.cs[c] * x[c - L.sy] + L.cs[c + L.sy] * x[c + L.sy] +
-                  L.cb[c] * x[c - L.sz] + L.cb[c + L.sz] * x[c + L.sz];
-    return (L.b[c] + off) / D;
+// colour values: exactly the two in-place colour passes of k_mg_smooth (same operations
+// in the same order, so the results are bit-equal).
+//
+// A workgroup owns a TX x TY column tile (one thread per column) and marches z through
+// its chunk of planes.  At plane step k every thread loads its own column's operator of
+// plane k+1 (cw, cw east, cs, cs north, g, b; cb and the iterate of plane k+2) whatever
+// its colour, relaxes the first colour of plane k+1 if its cell has that colour, and
+// keeps the values in registers until step k+1 relaxes the second colour there: each
+// cache line of the level is fetched once per sweep, and only in-plane neighbour reads
+// (east / north faces, the iterate's four in-plane neighbours) revisit lines, within the
+// same step.  The first colour is also relaxed on the one-cell ring around the tile (row
+// j0-1 by tile row 0, row j0+TY by tile row TY-1, the two side cells of each row by x
+// threads 0 and 1), from xo, so tiles never read each other's output; the second colour
+// of plane k reads the first-colour planes k-1, k, k+1 from LDS.  Four LDS slots and one
+// barrier per step: step k writes slot k+1 and reads slots k-1..k+1, so the slot that
+// step k+1 writes is idle, and a wave can only write slot k+3 = k-1 after every wave has
+// passed step k+1's barrier.
+//
+// Addresses are byte offsets of type O on the arrays' scalar base addresses (32-bit when
+// the level's arrays are under 2 GiB).  EAGER issues the neighbour and ring loads before
+// any of them is used (one wait on memory per plane step); otherwise they are read where
+// needed, which holds fewer registers (the FP64 form).
+template <typename T, typename O>
+__device__ __forceinline__ T ldb(const T* base, O off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
 }
 
-template <typename T, int TY>
-__global__ __launch_bounds__(BX * TY) void k_mg_rbsweep(Lv<T> L, const T* xo, T* xn, int c1, int kc, int gx, int gy,
-                                                        int gz, const SolveCtl* ctl) {
-    constexpr int RX = BX + 2, RY = TY + 2, NT = BX * TY;
+template <typename T, typename O, bool EAGER>
+__device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O dz) {
+    constexpr O E = sizeof(T);
+    const T g = ldb(L.g, c), cw = ldb(L.cw, c), ce = ldb(L.cw, c + E), cs = ldb(L.cs, c), cn = ldb(L.cs, c + dy),
+            cb = ldb(L.cb, c), ct = ldb(L.cb, c + dz), b = ldb(L.b, c);
+    const T D = g + cw + ce + cs + cn + cb + ct;
+    if (EAGER) {
+        const T x = ldb(xo, c), xw = ldb(xo, c - E), xe = ldb(xo, c + E), xs = ldb(xo, c - dy), xn = ldb(xo, c + dy),
+                xb = ldb(xo, c - dz), xt = ldb(xo, c + dz);
+        const T off = cw * xw + ce * xe + cs * xs + cn * xn + cb * xb + ct * xt;
+        const T rel = (b + off) / D;
+        return (D > T(0)) ? rel : x;
+    }
+    if (!(D > T(0))) return ldb(xo, c);
+    const T off = cw * ldb(xo, c - E) + ce * ldb(xo, c + E) + cs * ldb(xo, c - dy) + cn * ldb(xo, c + dy) +
+                  cb * ldb(xo, c - dz) + ct * ldb(xo, c + dz);
+    return (b + off) / D;
+}
+
+template <typename T, int TX, int TY, typename O, bool EAGER>
+__global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1,
+                                                        int kc, int gx, int gy, int gz, const SolveCtl* ctl) {
+    constexpr int RX = TX + 2, RY = TY + 2;
+    constexpr O E = sizeof(T);
     if (ctl->done) return;
-    __shared__ T X1[3][RY * RX];
+    __shared__ T X1[4][RY * RX];
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
-    const int i0 = 1 + tl.bx * BX, j0 = 1 + tl.by * TY;
+    const int i0 = 1 + tl.bx * TX, j0 = 1 + tl.by * TY;
     const int k0 = 1 + tl.bz * kc, k1 = min(L.nz, k0 + kc - 1);
-    const int tid = threadIdx.x + BX * threadIdx.y;
-    // first-colour values of plane k on the ring-1 region (0 outside the level)
-    auto first_plane = [&](int k) {
-        T* P = X1[k % 3];
-        for (int e = tid; e < RX * RY; e += NT) {
-            const int i = i0 - 1 + e % RX, j = j0 - 1 + e / RX;
-            T v = T(0);
-            if (k >= 1 && k <= L.nz && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && ((i + j + k) & 1) == c1)
-                v = relax_from(L, xo, i + L.sy * j + L.sz * k);
-            P[e] = v;
-        }
-    };
-    first_plane(k0 - 1);
-    first_plane(k0);
-    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = i0 + lane, j = j0 + w;
     const bool inside = i <= L.nx && j <= L.ny;
-    const int r = (threadIdx.y + 1) * RX + threadIdx.x + 1;
-    for (int k = k0; k <= k1; ++k) {
-        first_plane(k + 1);
-        __syncthreads();
+    const int r = (w + 1) * RX + lane + 1;
+    const O dy = (O)L.sy * E, dz = (O)L.sz * E;
+    const O col = (O)(i + L.sy * j) * E;   // own column, plane 0 (read only when inside)
+    const int rowring = (w == 0) ? j0 - 1 : (w == TY - 1 ? j0 + TY : -1);
+    const int rowslot = (w == 0) ? lane + 1 : (TY + 1) * RX + lane + 1;
+    const bool rowring_in = rowring >= 1 && rowring <= L.ny && i <= L.nx;
+    const int si = lane == 0 ? i0 - 1 : i0 + TX;
+    const int sslot = (w + 1) * RX + (lane == 0 ? 0 : TX + 1);
+    const bool side_in = lane < 2 && j <= L.ny && si >= 1 && si <= L.nx;
+
+    T cw0 = 0, ce0 = 0, cs0 = 0, cn0 = 0, g0 = 0, b0 = 0;   // plane k
+    T cw1 = 0, ce1 = 0, cs1 = 0, cn1 = 0, g1 = 0, b1 = 0;   // plane k+1
+    T cbk = 0, cbk1 = 0, cbk2 = 0;                          // cb of planes k, k+1, k+2
+    T xk = 0, xk1 = 0, xk2 = 0;                             // iterate of planes k, k+1, k+2
+    T xw = 0, xe = 0, xs = 0, xnn = 0;                      // iterate's in-plane neighbours, plane k+1
+    T own1 = 0, own1n = 0;                                  // first-colour value of the own cell, k / k+1
+    const int kfirst = k0 - 2;
+    if (inside) {
+        if (kfirst >= 0) xk = ldb(xo, col + (O)kfirst * dz);
+        xk1 = ldb(xo, col + (O)(k0 - 1) * dz);
+        cbk1 = ldb(L.cb, col + (O)(k0 - 1) * dz);
+    }
+    for (int k = kfirst; k <= k1; ++k) {
+        const int kp = k + 1;   // in [0, nz + 1]: planes of the padded level
         if (inside) {
-            const long c = i + L.sy * j + L.sz * k;
-            const T* P0 = X1[k % 3];
-            T out;
-            if (((i + j + k) & 1) == c1) {
-                out = P0[r];
-            } else {
-                const T D = diag_of(L, c);
-                if (!(D > T(0))) {
-                    out = xo[c];
-                } else {
-                    const T* Pm = X1[(k + 2) % 3];
-                    const T* Pp = X1[(k + 1) % 3];
-                    const T off = L.cw[c] * P0[r - 1] + L.cw[c + 1] * P0[r + 1] + L.cs[c] * P0[r - RX] +
-                                  L.cs[c + L.sy] * P0[r + RX] + L.cb[c] * Pm[r] + L.cb[c + L.sz] * Pp[r];
-                    out = (L.b[c] + off) / D;
-                }
+            const O c = col + (O)kp * dz;
+            cw1 = ldb(L.cw, c); ce1 = ldb(L.cw, c + E); cs1 = ldb(L.cs, c); cn1 = ldb(L.cs, c + dy);
+            g1 = ldb(L.g, c); b1 = ldb(L.b, c);
+            if (EAGER) {
+                xw = ldb(xo, c - E); xe = ldb(xo, c + E); xs = ldb(xo, c - dy); xnn = ldb(xo, c + dy);
             }
-            xn[c] = out;
+            if (kp + 1 <= L.nz + 1) {
+                cbk2 = ldb(L.cb, c + dz);
+                xk2 = ldb(xo, c + dz);
+            }
         }
-        __syncthreads();   // plane k-1's slot is rewritten by the next iteration
+        const bool kin = kp >= 1 && kp <= L.nz;
+        // first colour of plane k+1 on the ring, then on the own cell
+        if (rowring >= 0) {
+            T v = T(0);
+            if (rowring_in && kin && ((i + rowring + kp) & 1) == c1)
+                v = relax_at<T, O, EAGER>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
+            X1[kp & 3][rowslot] = v;
+        }
+        if (lane < 2) {
+            T v = T(0);
+            if (side_in && kin && ((si + j + kp) & 1) == c1)
+                v = relax_at<T, O, EAGER>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
+            X1[kp & 3][sslot] = v;
+        }
+        {
+            const bool on = inside && kin && ((i + j + kp) & 1) == c1;
+            if (!EAGER && on) {
+                const O c = col + (O)kp * dz;
+                xw = ldb(xo, c - E); xe = ldb(xo, c + E); xs = ldb(xo, c - dy); xnn = ldb(xo, c + dy);
+            }
+            const T D = g1 + cw1 + ce1 + cs1 + cn1 + cbk1 + cbk2;
+            const T off = cw1 * xw + ce1 * xe + cs1 * xs + cn1 * xnn + cbk1 * xk + cbk2 * xk2;
+            const T rel = (b1 + off) / D;
+            const T v = on ? ((D > T(0)) ? rel : xk1) : T(0);
+            X1[kp & 3][r] = v;
+            own1n = v;
+        }
+        // second colour of plane k
+        if (k >= k0) {
+            __syncthreads();
+            if (inside) {
+                const T* P0 = X1[k & 3];
+                const T* Pm = X1[(k - 1) & 3];
+                const T* Pp = X1[kp & 3];
+                const T D = g0 + cw0 + ce0 + cs0 + cn0 + cbk + cbk1;
+                const T off = cw0 * P0[r - 1] + ce0 * P0[r + 1] + cs0 * P0[r - RX] + cn0 * P0[r + RX] + cbk * Pm[r] +
+                              cbk1 * Pp[r];
+                const T rel = (b0 + off) / D;
+                const T out = (((i + j + k) & 1) == c1) ? own1 : ((D > T(0)) ? rel : xk);
+                *reinterpret_cast<T*>(reinterpret_cast<char*>(xn) + col + (O)k * dz) = out;
+            }
+        }
+        // plane k+1 becomes plane k
+        cw0 = cw1; ce0 = ce1; cs0 = cs1; cn0 = cn1; g0 = g1; b0 = b1;
+        cbk = cbk1; cbk1 = cbk2;
+        xk = xk1; xk1 = xk2;
+        own1 = own1n;
     }
 }
 
-template <typename T, int TY>
+template <typename T, int TX, int TY>
 int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof) {
-    const int gx = (L.nx + BX - 1) / BX, gy = (L.ny + TY - 1) / TY;
-    // about 1M threads in flight: 4096 workgroups of 256 threads, fewer of larger tiles
-    long want = (4096L * 256 / (BX * TY)) / ((long)gx * gy);
+    constexpr bool EAGER = sizeof(T) == sizeof(float);
+    const int gx = (L.nx + TX - 1) / TX, gy = (L.ny + TY - 1) / TY;
+    // deep z chunks (a chunk reads two planes beyond its own): about 512K threads in
+    // flight, i.e. 2 workgroups of 1024 threads per CU
+    long want = (1024L * 512 / (TX * TY)) / ((long)gx * gy);
     if (want < 1) want = 1;
     int kc = (int)((L.nz + want - 1) / want);
     if (kc < 1) kc = 1;
     const int gz = (L.nz + kc - 1) / kc;
+    const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
     const int slot = prof ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
-    k_mg_rbsweep<T, TY><<<dim3(gx * gy * gz), dim3(BX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
+    if (small)
+        k_mg_rbsweep<T, TX, TY, uint32_t, EAGER>
+            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
+    else
+        k_mg_rbsweep<T, TX, TY, uint64_t, EAGER>
+            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
     HS_TRY(prof_end(c, slot));
     return HS_OK;
 }
@@ -509,16 +595,20 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
     // fused out-of-place sweeps x -> x2; after each the level's two iterate buffers swap,
     // so m->x always holds the latest iterate (every user reads it at launch time)
     const int c1 = forward ? 0 : 1;
-    // tile rows of the fused sweep: 0 picks the measured best (8 in float, 16 in double)
-    const int rows = c->prm.mg_sweep_rows > 0 ? c->prm.mg_sweep_rows : (sizeof(T) == sizeof(float) ? 8 : 16);
-    if (sweeps > 0 && rows > 0) {
+    // tile of the fused sweep: 0 picks one by the level's width (256 x 4, 128 x 8 or
+    // 64 x 8 columns); 4, 8 or 16 force 64-wide tiles of that many rows; below 0 the two
+    // in-place colour passes run instead
+    const int rows = c->prm.mg_sweep_rows;
+    if (sweeps > 0 && rows >= 0) {
         for (; sweeps > 0; --sweeps) {
             const T* src = (T*)m->x;
             T* dst = (T*)m->x2;
             const bool prof = m->level == 0;
-            if (rows == 4) HS_TRY((launch_rbsweep<T, 4>(c, L, src, dst, c1, prof)));
-            else if (rows == 16) HS_TRY((launch_rbsweep<T, 16>(c, L, src, dst, c1, prof)));
-            else HS_TRY((launch_rbsweep<T, 8>(c, L, src, dst, c1, prof)));
+            if (rows == 4) HS_TRY((launch_rbsweep<T, 64, 4>(c, L, src, dst, c1, prof)));
+            else if (rows == 16) HS_TRY((launch_rbsweep<T, 64, 16>(c, L, src, dst, c1, prof)));
+            else if (rows > 0 || L.nx <= 64) HS_TRY((launch_rbsweep<T, 64, 8>(c, L, src, dst, c1, prof)));
+            else if (L.nx <= 128) HS_TRY((launch_rbsweep<T, 128, 8>(c, L, src, dst, c1, prof)));
+            else HS_TRY((launch_rbsweep<T, 256, 4>(c, L, src, dst, c1, prof)));
             std::swap(m->x, m->x2);
             L.x = (T*)m->x;
         }

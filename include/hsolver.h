@@ -104,8 +104,9 @@ typedef struct {
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
     int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
-    int32_t mg_sweep_rows;            /* rows (y) per tile of the fused red-black sweep: 4, 8 or 16;
-                                         0 = 8 for the float V-cycle, 16 for the double one */
+    int32_t mg_sweep_rows;            /* tile of the fused red-black sweep: 0 = by level width
+                                         (256x4, 128x8 or 64x8 columns); 4, 8, 16 = 64-wide tiles
+                                         of that many rows; -1 = two in-place colour passes */
     int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post);
                                          defaults -1 / 1, i.e. V(2,1) there */
     int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
