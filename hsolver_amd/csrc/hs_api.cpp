@@ -364,7 +364,7 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     A(&c->ptemp, d.nP); A(&c->pprime, d.nP); A(&c->tf, d.nP); A(&c->ts, d.nP); A(&c->b, d.nP);
     if (modeR(c)) { A(&c->ax, d.nP); A(&c->ay, d.nP); A(&c->az, d.nP); }
     else { A(&c->ax, d.nU); A(&c->ay, d.nV); A(&c->az, d.nW); }
-    A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
+    A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->kp2, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
     A(&c->dres, std::max(std::max(d.nU, d.nV), std::max(d.nW, d.nP)));
     A(&c->partials, 65536);
     if (e == HS_OK) e = dalloc(&c->cls, (size_t)d.nP);
@@ -432,8 +432,8 @@ void hs_destroy(hs_ctx* c) {
     gmg_free(c);
     energy_free(c);
     double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
-                      c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->ks, c->kx, c->kz,
-                      c->kq, c->kb, c->dres, c->partials};
+                      c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->kp2, c->ks, c->kx,
+                      c->kz, c->kq, c->kb, c->dres, c->partials};
     for (double* b : bufs) if (b) hipFree(b);
     if (c->cls) hipFree(c->cls);
     if (c->sctl) hipFree(c->sctl);

@@ -186,6 +186,7 @@ struct hs_ctx {
     double *ax = nullptr, *ay = nullptr, *az = nullptr;
     // Krylov work (P layout)
     double *kr = nullptr, *kp = nullptr, *ks = nullptr, *kx = nullptr, *kz = nullptr, *kq = nullptr;
+    double* kp2 = nullptr;   // second direction buffer of flexible CG (fused update + apply)
     double* kb = nullptr;               // right-hand side staged from the host
     // per-point squared change for lexicographic sweeps (max layout size)
     double* dres = nullptr;
@@ -252,6 +253,7 @@ int spmv(hs_ctx* c, const double* x, double* y);
 int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
 int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
 int cg_spmv_dot(hs_ctx* c, const double* p, double* q);
+int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* pn, double* q);
 int mask_fluid(hs_ctx* c, double* x);
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);
 int correct_velocity(hs_ctx* c);

@@ -295,26 +295,29 @@ __device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long
     return 1;
 }
 
-// Red-black colour pass over the face box [1..ni] x [1..nj] x [1..nk]; per-block
-// partial of the squared change; the finish kernel runs the control step after colour 1.
+// Red-black colour pass over the face box [1..ni] x [1..nj] x [1..nk], two adjacent faces
+// per thread along x: on every plane one of the two has the pass's colour, so every lane
+// updates a face at every k (one face per thread idles half the lanes of each plane);
+// tiles go to the XCDs in contiguous runs.  Per-block partial of the squared change; the
+// finish kernel runs the control step after colour 1.
 template <int COMP>
-__global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour,
-                                               double* partials, SweepCtl* wctl) {
+__global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour, int gx, int gy,
+                                                int gz, double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
     const int tid = threadIdx.x + BX * threadIdx.y;
-    const long i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long ib = 1 + (long)tl.bx * (2 * BX) + 2 * threadIdx.x, j = 1 + (long)tl.by * BY + threadIdx.y;
     double acc = 0.0;
-    if (i <= ni && j <= nj) {
-        const int k0 = 1 + blockIdx.z * kc, k1 = min(nk, k0 + kc - 1);
+    if (ib <= ni && j <= nj) {
+        const int k0 = 1 + tl.bz * kc, k1 = min(nk, k0 + kc - 1);
         for (int k = k0; k <= k1; ++k) {
-            if (((i + j + k) & 1) != colour) continue;
+            const long i = ib + ((((ib + j + k) & 1) != colour) ? 1 : 0);
+            if (i > ni) continue;
             double dd = 0.0;
             if (mom_update<COMP>(a, i, j, k, &dd) == 1) acc += dd;
         }
     }
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    put_partial<BT>(acc, tid, bid, partials + (colour ? nb : 0));
+    put_partial<BT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
 }
 
 // One hyperplane i+j+k = s of the lexicographic sweep; dres gets the squared change.
@@ -437,23 +440,25 @@ __device__ __forceinline__ bool t_update(const TArgs& a, long i, long j, long k,
     return true;
 }
 
+// Red-black colour pass of the energy sweep over a box, two adjacent cells per thread
+// along x (every lane busy on every plane), tiles in contiguous runs per XCD.
 __global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int kc, int colour,
-                                             double* partials, SweepCtl* wctl) {
+                                              int gx, int gy, int gz, double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
     const int tid = threadIdx.x + BX * threadIdx.y;
-    const long i = i0 + blockIdx.x * BX + threadIdx.x, j = j0 + blockIdx.y * BY + threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long ib = i0 + (long)tl.bx * (2 * BX) + 2 * threadIdx.x, j = j0 + (long)tl.by * BY + threadIdx.y;
     double acc = 0.0;
-    if (i < i0 + ni && j < j0 + nj) {
-        const int ka = k0 + blockIdx.z * kc, kb = min(k0 + nk - 1, ka + kc - 1);
+    if (ib < i0 + ni && j < j0 + nj) {
+        const int ka = k0 + tl.bz * kc, kb = min(k0 + nk - 1, ka + kc - 1);
         for (int k = ka; k <= kb; ++k) {
-            if (((i + j + k) & 1) != colour) continue;
+            const long i = ib + ((((ib + j + k) & 1) != colour) ? 1 : 0);
+            if (i >= i0 + ni) continue;
             double dd = 0.0;
             if (t_update(a, i, j, k, &dd)) acc += dd;
         }
     }
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    put_partial<BT>(acc, tid, bid, partials + (colour ? nb : 0));
+    put_partial<BT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
 }
 
 __global__ __launch_bounds__(256) void k_t_plane(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int s,
@@ -547,8 +552,9 @@ struct OpArgs {
 // of fluid_solver.h:2003-2083).  Entries of x outside fluid cells are zero.  xb/xc/xt
 // are x at k-1, k, k+1 and cb/ct the bottom/top z-face coefficients, carried in
 // registers by the z-marching callers.
-__device__ __forceinline__ double apply_row(const OpArgs& o, const double* x, long c, long cu, long cv,
-                                            double xb, double xc, double xt, double cB, double cT) {
+template <typename XA>
+__device__ __forceinline__ double apply_row(const OpArgs& o, XA x, long c, long cu, long cv, double xb, double xc,
+                                            double xt, double cB, double cT) {
     const Dims& d = o.d;
     double cS, cW, cE, cN, dg;
     if (o.modeR) {
@@ -561,27 +567,29 @@ __device__ __forceinline__ double apply_row(const OpArgs& o, const double* x, lo
     }
     double t = dg * xc;
     t += (-cB) * xb;
-    t += (-cS) * x[c - d.sPy];
-    t += (-cW) * x[c - 1];
-    t += (-cE) * x[c + 1];
-    t += (-cN) * x[c + d.sPy];
+    t += (-cS) * x(c - d.sPy);
+    t += (-cW) * x(c - 1);
+    t += (-cE) * x(c + 1);
+    t += (-cN) * x(c + d.sPy);
     t += (-cT) * xt;
     return t;
 }
 
-// Walks one (i,j) column over planes k0..k1 calling f(c, y) for fluid cells.
-template <typename F>
-__device__ __forceinline__ void spmv_column(const OpArgs& o, const double* x, long i, long j, int k0, int k1, F f) {
+// Walks one (i,j) column over planes k0..k1 calling f(c, y, x_c) for fluid cells and
+// g(c, x_c) for every cell; x(c) gives the operand at padded index c.
+template <typename XA, typename F, typename G>
+__device__ __forceinline__ void spmv_column(const OpArgs& o, XA x, long i, long j, int k0, int k1, F f, G g) {
     const Dims& d = o.d;
     long c = idxP(d, i, j, k0), cu = idxU(d, i, j, k0), cv = idxV(d, i, j, k0);
-    double xb = x[c - d.sPz], xc = x[c];
+    double xb = x(c - d.sPz), xc = x(c);
     // z-face coefficients: Mode R row coefficient az at the cell (both faces), Mode C the
     // W-layout faces k and k+1 (W shares the P strides)
     double cb = o.az[c];
     for (int k = k0; k <= k1; ++k, c += d.sPz, cu += d.sUz, cv += d.sVz) {
-        const double xt = x[c + d.sPz];
+        const double xt = x(c + d.sPz);
         const double ct = o.modeR ? o.az[c] : o.az[c + d.sPz];
-        if (o.cls[c] == CELL_FLUID) f(c, apply_row(o, x, c, cu, cv, xb, xc, xt, o.modeR ? ct : cb, ct));
+        g(c, xc);
+        if (o.cls[c] == CELL_FLUID) f(c, apply_row(o, x, c, cu, cv, xb, xc, xt, o.modeR ? ct : cb, ct), xc);
         xb = xc;
         xc = xt;
         cb = o.modeR ? o.az[c + d.sPz] : ct;
@@ -599,10 +607,41 @@ __global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, doubl
     double acc = 0.0;
     if (i <= d.nx && j <= d.ny) {
         const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
-        spmv_column(o, p, i, j, k0, k1, [&](long c, double y) {
-            s[c] = y;
-            acc += y * p[c];
-        });
+        spmv_column(
+            o, [p](long q) { return p[q]; }, i, j, k0, k1,
+            [&](long c, double y, double pc) {
+                s[c] = y;
+                acc += y * pc;
+            },
+            [](long, double) {});
+    }
+    put_partial<BT>(acc, tid, blockIdx.x, partials);
+}
+
+// Flexible CG direction update fused into the operator apply: p_new = z + beta p_old is
+// formed at every point the stencil reads (the same expression as a separate update pass,
+// so the same values), written to pn for the column's cells, and q = A p_new with block
+// partials of <p_new, q>.  pn must not alias p (neighbouring columns still read p_old).
+template <typename T>
+__global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__ z, const double* __restrict__ p,
+                                                 double* __restrict__ pn, double* __restrict__ s, int kc, int gx, int gy,
+                                                 int gz, double* partials, SolveCtl* ctl) {
+    if (ctl->done) return;
+    const Dims& d = o.d;
+    const double beta = ctl->beta;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    double acc = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
+        spmv_column(
+            o, [=](long q) { return (double)z[q] + beta * p[q]; }, i, j, k0, k1,
+            [&](long c, double y, double pc) {
+                s[c] = y;
+                acc += y * pc;
+            },
+            [=](long c, double pc) { pn[c] = pc; });
     }
     put_partial<BT>(acc, tid, blockIdx.x, partials);
 }
@@ -711,7 +750,8 @@ __global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* 
     if (i > d.nx || j > d.ny) return;
     const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
     for (int k = k0; k <= k1; ++k) y[idxP(d, i, j, k)] = 0.0;
-    spmv_column(o, x, i, j, k0, k1, [&](long c, double v) { y[c] = v; });
+    spmv_column(
+        o, [x](long q) { return x[q]; }, i, j, k0, k1, [&](long c, double v, double) { y[c] = v; }, [](long, double) {});
 }
 
 // zero entries outside fluid cells
@@ -933,18 +973,18 @@ int momentum_sweep(hs_ctx* c, int comp) {
     int ni = d.nx, nj = d.ny, nk = d.nz;
     if (!R) { if (comp == 0) ni += 1; else if (comp == 1) nj += 1; else nk += 1; }
     if (c->prm.order == HS_ORDER_RB) {
-        const long tiles = (long)((ni + BX - 1) / BX) * ((nj + BY - 1) / BY);
-        const int kc = kchunk(nk, tiles);
-        const dim3 gr = grid3(ni, nj, nk, kc);
+        const int gx = (ni + 2 * BX - 1) / (2 * BX), gy = (nj + BY - 1) / BY;
+        const int kc = kchunk(nk, (long)gx * gy);
+        const int gz = (nk + kc - 1) / kc;
         for (int colour = 0; colour < 2; ++colour) {
             const int slot = comp == 0 ? prof_begin(c, HS_K_MOM_RB) : -1;
-            if (comp == 0) k_mom_rb<0><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
-            else if (comp == 1) k_mom_rb<1><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
-            else k_mom_rb<2><<<gr, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, c->partials, c->wctl);
+            if (comp == 0) k_mom_rb<0><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
+            else if (comp == 1) k_mom_rb<1><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
+            else k_mom_rb<2><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
             HS_TRY(prof_end(c, slot));
         }
-        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * (int)(gr.x * gr.y * gr.z), nullptr));
+        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * gx * gy * gz, nullptr));
     } else {
         const long nl = comp == 0 ? d.nU : (comp == 1 ? d.nV : d.nW);
         HS_TRY(launch_fill(c, c->dres, nl, 0.0));
@@ -967,14 +1007,15 @@ static int t_sweep_box(hs_ctx* c, bool solid, int i0, int j0, int k0, int ni, in
     const Dims& d = c->d;
     TArgs a = t_args(c, solid);
     if (c->prm.order == HS_ORDER_RB) {
-        const long tiles = (long)((ni + BX - 1) / BX) * ((nj + BY - 1) / BY);
-        const int kc = kchunk(nk, tiles);
-        const dim3 gr = grid3(ni, nj, nk, kc);
+        const int gx = (ni + 2 * BX - 1) / (2 * BX), gy = (nj + BY - 1) / BY;
+        const int kc = kchunk(nk, (long)gx * gy);
+        const int gz = (nk + kc - 1) / kc;
         for (int colour = 0; colour < 2; ++colour) {
-            k_t_rb<<<gr, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, c->partials, c->wctl);
+            k_t_rb<<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, gx, gy, gz,
+                                                                 c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
         }
-        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * (int)(gr.x * gr.y * gr.z), nullptr));
+        HS_TRY(launch_finish(c, FIN_SWEEP, 2 * gx * gy * gz, nullptr));
     } else {
         HS_TRY(launch_fill(c, c->dres, d.nP, 0.0));
         const long njk = (long)nj * nk;
@@ -1141,6 +1182,24 @@ int mask_fluid(hs_ctx* c, double* x) {
     k_mask_fluid<<<1024, 256, 0, c->stream>>>(c->d.nP, c->cls, x);
     HS_CHECK(hipGetLastError());
     return HS_OK;
+}
+
+// pn = z + beta p, q = A pn with <pn,q> and alpha (flexible CG, iterations after the first)
+int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* pn, double* q) {
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    const int slot = prof_begin(c, HS_K_SPMV);
+    if (zf32)
+        k_fcg_spmv<float><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const float*)z, p, pn, q, kc, gr.x,
+                                                                      gr.y, gr.z, c->partials, c->sctl);
+    else
+        k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const double*)z, p, pn, q, kc, gr.x,
+                                                                       gr.y, gr.z, c->partials, c->sctl);
+    HS_CHECK(hipGetLastError());
+    HS_TRY(prof_end(c, slot));
+    return launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr);
 }
 
 // q = A p with <p,q> and alpha = gamma / <p,q> in the last block (shared by CG and FCG).

@@ -164,15 +164,24 @@ __device__ __forceinline__ T ldb(const T* base, O off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
 }
 
-template <typename T, typename O, bool EAGER>
+// the old iterate at byte offset off; ZX: the iterate is zero everywhere (first sweep
+// from a zero guess), read as the constant and still multiplied, so the arithmetic (and
+// every result bit) is that of a sweep over a zero-filled buffer
+template <typename T, typename O, bool ZX>
+__device__ __forceinline__ T ldx(const T* xo, O off) {
+    return ZX ? T(0) : ldb(xo, off);
+}
+
+template <typename T, typename O, bool EAGER, bool ZX>
 __device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O dz) {
     constexpr O E = sizeof(T);
     const T g = ldb(L.g, c), cw = ldb(L.cw, c), ce = ldb(L.cw, c + E), cs = ldb(L.cs, c), cn = ldb(L.cs, c + dy),
             cb = ldb(L.cb, c), ct = ldb(L.cb, c + dz), b = ldb(L.b, c);
     const T D = g + cw + ce + cs + cn + cb + ct;
-    if (EAGER) {
-        const T x = ldb(xo, c), xw = ldb(xo, c - E), xe = ldb(xo, c + E), xs = ldb(xo, c - dy), xn = ldb(xo, c + dy),
-                xb = ldb(xo, c - dz), xt = ldb(xo, c + dz);
+    if (EAGER || ZX) {
+        const T x = ldx<T, O, ZX>(xo, c), xw = ldx<T, O, ZX>(xo, c - E), xe = ldx<T, O, ZX>(xo, c + E),
+                xs = ldx<T, O, ZX>(xo, c - dy), xn = ldx<T, O, ZX>(xo, c + dy), xb = ldx<T, O, ZX>(xo, c - dz),
+                xt = ldx<T, O, ZX>(xo, c + dz);
         const T off = cw * xw + ce * xe + cs * xs + cn * xn + cb * xb + ct * xt;
         const T rel = (b + off) / D;
         return (D > T(0)) ? rel : x;
@@ -183,7 +192,7 @@ __device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O 
     return (b + off) / D;
 }
 
-template <typename T, int TX, int TY, typename O, bool EAGER>
+template <typename T, int TX, int TY, typename O, bool EAGER, bool ZX>
 __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1,
                                                         int kc, int gx, int gy, int gz, const SolveCtl* ctl) {
     constexpr int RX = TX + 2, RY = TY + 2;
@@ -214,8 +223,8 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
     T own1 = 0, own1n = 0;                                  // first-colour value of the own cell, k / k+1
     const int kfirst = k0 - 2;
     if (inside) {
-        if (kfirst >= 0) xk = ldb(xo, col + (O)kfirst * dz);
-        xk1 = ldb(xo, col + (O)(k0 - 1) * dz);
+        if (kfirst >= 0) xk = ldx<T, O, ZX>(xo, col + (O)kfirst * dz);
+        xk1 = ldx<T, O, ZX>(xo, col + (O)(k0 - 1) * dz);
         cbk1 = ldb(L.cb, col + (O)(k0 - 1) * dz);
     }
     for (int k = kfirst; k <= k1; ++k) {
@@ -224,12 +233,13 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
             const O c = col + (O)kp * dz;
             cw1 = ldb(L.cw, c); ce1 = ldb(L.cw, c + E); cs1 = ldb(L.cs, c); cn1 = ldb(L.cs, c + dy);
             g1 = ldb(L.g, c); b1 = ldb(L.b, c);
-            if (EAGER) {
-                xw = ldb(xo, c - E); xe = ldb(xo, c + E); xs = ldb(xo, c - dy); xnn = ldb(xo, c + dy);
+            if (EAGER || ZX) {
+                xw = ldx<T, O, ZX>(xo, c - E); xe = ldx<T, O, ZX>(xo, c + E);
+                xs = ldx<T, O, ZX>(xo, c - dy); xnn = ldx<T, O, ZX>(xo, c + dy);
             }
             if (kp + 1 <= L.nz + 1) {
                 cbk2 = ldb(L.cb, c + dz);
-                xk2 = ldb(xo, c + dz);
+                xk2 = ldx<T, O, ZX>(xo, c + dz);
             }
         }
         const bool kin = kp >= 1 && kp <= L.nz;
@@ -237,18 +247,18 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
         if (rowring >= 0) {
             T v = T(0);
             if (rowring_in && kin && ((i + rowring + kp) & 1) == c1)
-                v = relax_at<T, O, EAGER>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
+                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
             X1[kp & 3][rowslot] = v;
         }
         if (lane < 2) {
             T v = T(0);
             if (side_in && kin && ((si + j + kp) & 1) == c1)
-                v = relax_at<T, O, EAGER>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
+                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
             X1[kp & 3][sslot] = v;
         }
         {
             const bool on = inside && kin && ((i + j + kp) & 1) == c1;
-            if (!EAGER && on) {
+            if (!EAGER && !ZX && on) {
                 const O c = col + (O)kp * dz;
                 xw = ldb(xo, c - E); xe = ldb(xo, c + E); xs = ldb(xo, c - dy); xnn = ldb(xo, c + dy);
             }
@@ -282,8 +292,8 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
     }
 }
 
-template <typename T, int TX, int TY>
-int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof) {
+template <typename T, int TX, int TY, bool ZX>
+int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof) {
     constexpr bool EAGER = sizeof(T) == sizeof(float);
     const int gx = (L.nx + TX - 1) / TX, gy = (L.ny + TY - 1) / TY;
     // deep z chunks (a chunk reads two planes beyond its own): about 512K threads in
@@ -296,13 +306,20 @@ int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool
     const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
     const int slot = prof ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
     if (small)
-        k_mg_rbsweep<T, TX, TY, uint32_t, EAGER>
+        k_mg_rbsweep<T, TX, TY, uint32_t, EAGER, ZX>
             <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
     else
-        k_mg_rbsweep<T, TX, TY, uint64_t, EAGER>
+        k_mg_rbsweep<T, TX, TY, uint64_t, EAGER, ZX>
             <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
     HS_TRY(prof_end(c, slot));
     return HS_OK;
+}
+
+// zx: the iterate is zero (first sweep from a zero guess; the buffer is not read)
+template <typename T, int TX, int TY>
+int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof, bool zx) {
+    return zx ? launch_rbsweep_z<T, TX, TY, true>(c, L, src, dst, c1, prof)
+              : launch_rbsweep_z<T, TX, TY, false>(c, L, src, dst, c1, prof);
 }
 
 // b_c = sum over active children of (b - A x); x_c = 0.
@@ -310,7 +327,7 @@ int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool
 // the residuals of its two fine cells of coarse plane K, and the 2 x 2 in-plane sums go
 // through LDS.
 template <typename T>
-__global__ __launch_bounds__(BT) void k_mg_restrict(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz,
+__global__ __launch_bounds__(BT) void k_mg_restrict(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz, int zero_x,
                                                    const SolveCtl* ctl) {
     if (ctl->done) return;
     __shared__ T red[BY][BX];
@@ -339,7 +356,7 @@ __global__ __launch_bounds__(BT) void k_mg_restrict(Lv<T> F, Lv<T> C, int kc, in
             const T s = (red[2 * cj][2 * ci] + red[2 * cj][2 * ci + 1]) + (red[2 * cj + 1][2 * ci] + red[2 * cj + 1][2 * ci + 1]);
             const long cc = I + C.sy * J + C.sz * K;
             C.b[cc] = s;
-            C.x[cc] = T(0);
+            if (zero_x) C.x[cc] = T(0);
         }
         __syncthreads();
     }
@@ -392,14 +409,16 @@ inline TileShape tile_shape(int nx, int ny, int nz) {
     return s;
 }
 
+// zero_x: also clear the coarse iterate (not needed when its first sweep reads it as zero)
 template <typename T>
-int launch_restrict(hs_ctx* c, const Lv<T>& F, const Lv<T>& C) {
+int launch_restrict(hs_ctx* c, const Lv<T>& F, const Lv<T>& C, bool zero_x) {
     // fine-cell tiles in x and y, coarse planes in z
     const long tiles = (long)((F.nx + BX - 1) / BX) * ((F.ny + BY - 1) / BY);
     TileShape s;
     s.kc = hsd::kchunk(C.nz, tiles);
     s.gr = dim3((F.nx + BX - 1) / BX, (F.ny + BY - 1) / BY, (C.nz + s.kc - 1) / s.kc);
-    k_mg_restrict<T><<<hsd::grid1(s.gr), dim3(BX, BY), 0, c->stream>>>(F, C, s.kc, s.gr.x, s.gr.y, s.gr.z, c->sctl);
+    k_mg_restrict<T><<<hsd::grid1(s.gr), dim3(BX, BY), 0, c->stream>>>(F, C, s.kc, s.gr.x, s.gr.y, s.gr.z, zero_x ? 1 : 0,
+                                                                      c->sctl);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
@@ -506,14 +525,14 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(LvPack<T> P, int t, in
 // x = 0, r = b, level-0 right-hand side = (T) b, level-0 iterate = 0; partials of <b,b>
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, double* x, double* r, T* b0, T* x0,
-                                                    double* partials) {
+                                                    int zero_x0, double* partials) {
     double acc = 0.0;
     GRID_STRIDE(t, n) {
         const double bv = b[t];
         x[t] = 0.0;
         r[t] = bv;
         b0[t] = (T)bv;
-        x0[t] = T(0);
+        if (zero_x0) x0[t] = T(0);
         acc += bv * bv;
     }
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
@@ -533,10 +552,11 @@ __global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, con
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// x += alpha p, r -= alpha q; level-0 right-hand side = (T) r, iterate = 0; partials of <r,r>
+// x += alpha p, r -= alpha q; level-0 right-hand side = (T) r, iterate = 0 (unless the
+// first sweep reads it as zero); partials of <r,r>
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const double* p, const double* q,
-                                                     T* b0, T* x0, double* partials, const SolveCtl* ctl) {
+                                                     T* b0, T* x0, int zero_x0, double* partials, const SolveCtl* ctl) {
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double acc = 0.0;
@@ -546,7 +566,7 @@ __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* 
         r[t] = rv;
         acc += rv * rv;
         b0[t] = (T)rv;
-        x0[t] = T(0);
+        if (zero_x0) x0[t] = T(0);
     }
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
@@ -565,14 +585,6 @@ __global__ __launch_bounds__(FLAT) void k_fcg_beta(long n, const T* z, const dou
     hsd::put_partial2<FLAT>(a, b, threadIdx.x, blockIdx.x, gridDim.x, partials);
 }
 
-// p = z + beta p
-template <typename T>
-__global__ __launch_bounds__(FLAT) void k_fcg_pupdate(long n, const T* z, double* p, const SolveCtl* ctl) {
-    if (ctl->done) return;
-    const double beta = ctl->beta;
-    GRID_STRIDE(t, n) p[t] = (double)z[t] + beta * p[t];
-}
-
 // ------------------------------------------------------------ host side ---
 template <typename T>
 int alloc_level(MGLevel* m) {
@@ -586,8 +598,10 @@ int alloc_level(MGLevel* m) {
     return HS_OK;
 }
 
+// zx: the level's iterate is zero on entry and the first fused sweep does not read it
+// (the callers then skip writing those zeros)
 template <typename T>
-int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
+int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
     Lv<T> L = lv<T>(m);
     const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
     const int kc = hsd::kchunk(L.nz, tiles);
@@ -604,11 +618,12 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
             const T* src = (T*)m->x;
             T* dst = (T*)m->x2;
             const bool prof = m->level == 0;
-            if (rows == 4) HS_TRY((launch_rbsweep<T, 64, 4>(c, L, src, dst, c1, prof)));
-            else if (rows == 16) HS_TRY((launch_rbsweep<T, 64, 16>(c, L, src, dst, c1, prof)));
-            else if (rows > 0 || L.nx <= 64) HS_TRY((launch_rbsweep<T, 64, 8>(c, L, src, dst, c1, prof)));
-            else if (L.nx <= 128) HS_TRY((launch_rbsweep<T, 128, 8>(c, L, src, dst, c1, prof)));
-            else HS_TRY((launch_rbsweep<T, 256, 4>(c, L, src, dst, c1, prof)));
+            if (rows == 4) HS_TRY((launch_rbsweep<T, 64, 4>(c, L, src, dst, c1, prof, zx)));
+            else if (rows == 16) HS_TRY((launch_rbsweep<T, 64, 16>(c, L, src, dst, c1, prof, zx)));
+            else if (rows > 0 || L.nx <= 64) HS_TRY((launch_rbsweep<T, 64, 8>(c, L, src, dst, c1, prof, zx)));
+            else if (L.nx <= 128) HS_TRY((launch_rbsweep<T, 128, 8>(c, L, src, dst, c1, prof, zx)));
+            else HS_TRY((launch_rbsweep<T, 256, 4>(c, L, src, dst, c1, prof, zx)));
+            zx = false;
             std::swap(m->x, m->x2);
             L.x = (T*)m->x;
         }
@@ -626,6 +641,9 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward) {
 
 // sweeps before / after the coarse correction on level l (the finest level may differ)
 inline int pre_sweeps(const hs_ctx* c, int l) { return (l == 0 && c->prm.mg_pre0 >= 0) ? c->prm.mg_pre0 : c->prm.mg_pre; }
+// a level smoothed by the fused sweep starts its cycle visit from a zero guess that the
+// first sweep does not read, so nothing needs to write those zeros
+inline bool zero_guess(const hs_ctx* c, int l) { return c->prm.mg_sweep_rows >= 0 && pre_sweeps(c, l) > 0; }
 inline int post_sweeps(const hs_ctx* c, int l) {
     return (l == 0 && c->prm.mg_post0 >= 0) ? c->prm.mg_post0 : c->prm.mg_post;
 }
@@ -638,8 +656,9 @@ inline int post_sweeps(const hs_ctx* c, int l) {
 // 2: W-cycle below the finest level).  The second visit starts from the first's
 // iterate: restriction forms the residual of the current iterate, the coarse correction
 // is added by prolongation.
+// zx: level l's iterate is zero on entry (its first sweep does not read it)
 template <typename T>
-int cycle(hs_ctx* c, int l, int lend, bool dense) {
+int cycle(hs_ctx* c, int l, int lend, bool dense, bool zx) {
     const int nl = (int)c->mg.size();
     if (l == lend) {
         if (dense) return dense_solve(c, c->mg[l], sizeof(T) == sizeof(float));
@@ -650,25 +669,38 @@ int cycle(hs_ctx* c, int l, int lend, bool dense) {
         HS_CHECK(hipGetLastError());
         return HS_OK;
     }
-    HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true));
-    HS_TRY(launch_restrict<T>(c, lv<T>(c->mg[l]), lv<T>(c->mg[l + 1])));
+    HS_TRY(smooth<T>(c, c->mg[l], pre_sweeps(c, l), true, zx));
+    const bool zc = l + 1 != lend && zero_guess(c, l + 1);
+    HS_TRY(launch_restrict<T>(c, lv<T>(c->mg[l]), lv<T>(c->mg[l + 1]), !zc));
     const int reps = (l == 0 || l + 1 == lend) ? 1 : std::max(1, c->prm.mg_cycle);
-    for (int r = 0; r < reps; ++r) HS_TRY(cycle<T>(c, l + 1, lend, dense));
+    for (int r = 0; r < reps; ++r) HS_TRY(cycle<T>(c, l + 1, lend, dense, zc && r == 0));
     HS_TRY(launch_prolong<T>(c, lv<T>(c->mg[l]), lv<T>(c->mg[l + 1])));
     return smooth<T>(c, c->mg[l], post_sweeps(c, l), false);
 }
 
 // One cycle on the level-0 arrays: b holds the residual, x starts at 0 and leaves holding
 // the preconditioned residual (both prepared / consumed by the FCG kernels).
-template <typename T>
-int vcycle(hs_ctx* c) {
+// level solved directly by the cycle: the dense level, or the first one small enough for
+// the single-workgroup tail
+inline int cycle_end(const hs_ctx* c, bool* dense) {
     const int nl = (int)c->mg.size();
-    if (c->prm.mg_dense_coarse > 0 && c->dense) return cycle<T>(c, 0, nl - 1, true);
-    // first level small enough for the single-workgroup tail
+    *dense = c->prm.mg_dense_coarse > 0 && c->dense;
+    if (*dense) return nl - 1;
     int t = 0;
     while (t < nl && (long)c->mg[t]->nx * c->mg[t]->ny * c->mg[t]->nz > TAIL_MAX_CELLS) ++t;
-    if (t >= nl) t = nl - 1;
-    return cycle<T>(c, 0, t, false);
+    return t >= nl ? nl - 1 : t;
+}
+// the cycle's level-0 iterate is read as zero by its first sweep (nothing clears it)
+inline bool cycle_zero_guess(const hs_ctx* c) {
+    bool dense;
+    return cycle_end(c, &dense) > 0 && zero_guess(c, 0);
+}
+
+template <typename T>
+int vcycle(hs_ctx* c) {
+    bool dense;
+    const int t = cycle_end(c, &dense);
+    return cycle<T>(c, 0, t, dense, cycle_zero_guess(c));
 }
 
 template <typename T>
@@ -724,13 +756,14 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     h->maxit = c->prm.p_maxit;
     HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    double *r = c->kr, *p = c->kp, *q = c->ks;
+    double *r = c->kr, *p = c->kp, *p2 = c->kp2, *q = c->ks;
     T* b0 = (T*)c->mg[0]->b;
     // the level-0 iterate (the V-cycle's output z) moves between two buffers with the
     // smoother's sweeps: read its current pointer at every use
     auto z = [&]() { return (T*)c->mg[0]->x; };
     const int G = FLAT_BLOCKS;
-    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), c->partials);
+    const int zero_x0 = cycle_zero_guess(c) ? 0 : 1;
+    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->partials);
     HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));
     HS_TRY(vcycle<T>(c));
     k_fcg_start<T><<<G, FLAT, 0, c->stream>>>(n, r, z(), p, c->partials, c->sctl);
@@ -740,13 +773,19 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     int batch = c->last_iters > 2 ? c->last_iters - 1 : 4;
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
-            HS_TRY(cg_spmv_dot(c, p, q));
-            k_fcg_update<T><<<G, FLAT, 0, c->stream>>>(n, x, r, p, q, b0, z(), c->partials, c->sctl);
+            // the first direction is p = z (k_fcg_start); later ones are formed by the
+            // fused update + apply into the second buffer, and the buffers swap
+            if (launched == 0) {
+                HS_TRY(cg_spmv_dot(c, p, q));
+            } else {
+                HS_TRY(fcg_spmv_dot(c, z(), sizeof(T) == sizeof(float), p, p2, q));
+                std::swap(p, p2);
+            }
+            k_fcg_update<T><<<G, FLAT, 0, c->stream>>>(n, x, r, p, q, b0, z(), zero_x0, c->partials, c->sctl);
             HS_TRY(launch_finish(c, FIN_FCG_UPDATE, G, nullptr));
             HS_TRY(vcycle<T>(c));
             k_fcg_beta<T><<<G, FLAT, 0, c->stream>>>(n, z(), r, q, c->partials, c->sctl);
             HS_TRY(launch_finish(c, FIN_FCG_BETA, G, nullptr));
-            k_fcg_pupdate<T><<<G, FLAT, 0, c->stream>>>(n, z(), p, c->sctl);
         }
         HS_CHECK(hipGetLastError());
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
