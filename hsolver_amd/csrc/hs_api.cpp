@@ -143,22 +143,29 @@ int apply_bc_comp(hs_ctx* c, int comp) {
 }
 
 // Runs a Gauss-Seidel loop whose stop rule lives in wctl.  The device gates every sweep
-// on wctl->active, so when the tolerance is 0 (fixed sweep count) all sweeps are queued
-// without a host round trip; otherwise the host checks after each sweep.
+// (and its control step) on wctl->active, and the boundary kernels after a sweep are
+// idempotent, so a sweep queued past the stop changes nothing.  The host therefore queues
+// all sweeps when the count is fixed (tolerance 0), and otherwise as many as this loop
+// (`id`) ran last time before its first read of the control block, then one at a time.
 template <typename SweepFn, typename AfterFn>
-int gs_loop(hs_ctx* c, double n, double tol, int maxsteps, int quirk, SweepFn sweep, AfterFn after,
+int gs_loop(hs_ctx* c, int id, double n, double tol, int maxsteps, int quirk, SweepFn sweep, AfterFn after,
             int* sweeps, double* resid) {
     HS_TRY(sweep_begin(c, n, tol, maxsteps, quirk));
     const bool fixed = !(tol > 0.0) && quirk == 0;
-    for (int s = 0; s < maxsteps; ++s) {
+    const int first = fixed ? maxsteps : std::min(maxsteps, std::max(1, c->gs_last[id]));
+    int s = 0;
+    for (; s < first; ++s) {
         HS_TRY(sweep());
         HS_TRY(after());
-        if (!fixed) {
-            HS_TRY(read_wctl(c));
-            if (!c->h_wctl->active) break;
-        }
     }
     HS_TRY(read_wctl(c));
+    while (c->h_wctl->active && s < maxsteps) {
+        HS_TRY(sweep());
+        HS_TRY(after());
+        ++s;
+        HS_TRY(read_wctl(c));
+    }
+    c->gs_last[id] = c->h_wctl->step;
     if (sweeps) *sweeps = c->h_wctl->step;
     if (resid) *resid = c->h_wctl->global;
     if (std::isnan(c->h_wctl->global)) return HS_E_NAN;
@@ -178,16 +185,16 @@ int do_momentum(hs_ctx* c, int comp, int* sweeps, double* resid) {
         quirk = 0;
     }
     return gs_loop(
-        c, n, c->prm.mom_tol, c->prm.mom_sweeps_max, quirk, [&] { return momentum_sweep(c, comp); },
+        c, comp, n, c->prm.mom_tol, c->prm.mom_sweeps_max, quirk, [&] { return momentum_sweep(c, comp); },
         [&] { return apply_bc_comp(c, comp); }, sweeps, resid);
 }
 
 int do_energy(hs_ctx* c, int* sweeps, double* resid) {
     const Dims& d = c->d;
     if (modeR(c))
-        return gs_loop(c, (double)d.nx * d.ny * d.nz, c->prm.t_tol, c->prm.t_sweeps_max, 1,
+        return gs_loop(c, 3, (double)d.nx * d.ny * d.nz, c->prm.t_tol, c->prm.t_sweeps_max, 1,
                        [&] { return energy_sweep(c); }, [&] { return HS_OK; }, sweeps, resid);
-    return gs_loop(c, std::max(1.0, c->n_tcells), c->prm.t_tol, c->prm.t_sweeps_max, 0,
+    return gs_loop(c, 3, std::max(1.0, c->n_tcells), c->prm.t_tol, c->prm.t_sweeps_max, 0,
                    [&] { return energy_sweep(c); }, [&] { return bc_cfg_outflow(c, 3); }, sweeps, resid);
 }
 
@@ -522,7 +529,7 @@ int hs_solid_solve(hs_ctx* c, hs_stats* st) {
     int s = 0; double r = 0;
     if (modeR(c) && c->solid_regions > 0) {
         const double n = (double)(c->sb[1] - c->sb[0] + 1) * (c->sb[3] - c->sb[2] + 1) * (c->sb[5] - c->sb[4] + 1);
-        HS_TRY(gs_loop(c, n, c->prm.ts_tol, c->prm.ts_sweeps_max, 2, [&] { return solid_sweep(c); },
+        HS_TRY(gs_loop(c, 4, n, c->prm.ts_tol, c->prm.ts_sweeps_max, 2, [&] { return solid_sweep(c); },
                        [&] { return HS_OK; }, &s, &r));
     }
     if (st) { st->ts_sweeps = s; st->ts_residual = r; }

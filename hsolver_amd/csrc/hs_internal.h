@@ -172,6 +172,7 @@ struct hs_ctx {
     int sb[6] = {0, -1, 0, -1, 0, -1};  // solid bbox (padded, inclusive)
     double n_solved[3] = {0, 0, 0};     // Mode C solved-face counts
     double n_tcells = 0;                // Mode C energy cells (fluid + solid)
+    int gs_last[5] = {0, 0, 0, 0, 0};   // sweeps the last Gauss-Seidel loop of each kind ran
 
     // device geometry and classes
     double* geo = nullptr;              // 9 arrays of length max(n)+3 packed

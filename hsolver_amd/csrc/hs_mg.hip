@@ -770,7 +770,9 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
     HS_TRY(launch_finish(c, FIN_FCG_START, G, nullptr));
     HS_CHECK(hipGetLastError());
     int launched = 0;
-    int batch = c->last_iters > 2 ? c->last_iters - 1 : 4;
+    // first batch: as many iterations as the last solve took (an iteration queued after
+    // convergence exits at once on ctl->done), then two at a time
+    int batch = c->last_iters > 0 ? c->last_iters : 4;
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
             // the first direction is p = z (k_fcg_start); later ones are formed by the
