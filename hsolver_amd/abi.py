@@ -105,14 +105,14 @@ def params_default(mode: int) -> Params:
     p.precond = HS_PRECOND_NONE
     p.precision = HS_PREC_FP64
     p.mg_levels = 0
-    p.mg_pre = 2
-    p.mg_post = 2
+    p.mg_pre = -1
+    p.mg_post = -1
     p.mg_coarse_sweeps = 40
     p.mg_dense_coarse = 0
     p.mg_dense_refine = 2
     p.mg_sweep_rows = 0
     p.mg_pre0 = -1
-    p.mg_post0 = 1
+    p.mg_post0 = -1
     p.mg_cycle = 1
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0

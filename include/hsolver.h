@@ -94,7 +94,9 @@ typedef struct {
     int32_t precond;                  /* HS_PRECOND_* */
     int32_t precision;                /* HS_PREC_* */
     int32_t mg_levels;                /* multigrid levels including the finest; 0 = down to <= 4 cells a side */
-    int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction */
+    int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction; -1 (default)
+                                         = by grid: V(2,2) and V(2,1) on the finest level when no
+                                         cell's aspect ratio exceeds 8, else V(6,6) everywhere */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
     int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense FP32
                                          inverse built on the matrix cores, rebuilt every N hierarchy
@@ -107,8 +109,8 @@ typedef struct {
     int32_t mg_sweep_rows;            /* tile of the fused red-black sweep: 0 = by level width
                                          (256x4, 128x8 or 64x8 columns); 4, 8, 16 = 64-wide tiles
                                          of that many rows; -1 = two in-place colour passes */
-    int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post);
-                                         defaults -1 / 1, i.e. V(2,1) there */
+    int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post, or
+                                         the by-grid choice above) */
     int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
     int32_t reserved[2];
 } hs_params;
