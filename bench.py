@@ -181,7 +181,7 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.n),
-                "kernel": "k_mg_rbsweep<float> (finest level, one fused red-black sweep)",
+                "kernel": "k_mg_rbsweep<float,256,4> (finest level, one full fused red-black sweep)",
                 "launches": n_launch, "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": bytes_per_launch}
 

@@ -255,7 +255,8 @@ int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem);
 
 /* ---- live kernel timing (no counterpart in the reference; its only timer is clock()
  *      around outer iterations, solver.h:53-54,156-158) ---- */
-#define HS_K_MG_SMOOTH0  1   /* one red-black Gauss-Seidel sweep (fused colours) on the finest multigrid level */
+#define HS_K_MG_SMOOTH0  1   /* one red-black Gauss-Seidel sweep (fused colours) on the finest multigrid level,
+                                 from a nonzero iterate (the zero-guess first sweep is not counted) */
 #define HS_K_SPMV        2   /* FP64 p' operator apply + <p,q> of the Krylov iteration */
 #define HS_K_MOM_RB      3   /* one red-black colour pass of x-momentum */
 #define HS_K_E_SMOOTH0   4   /* one red-black colour pass on the finest energy multigrid level */
