@@ -190,3 +190,41 @@ def test_gmg_simple_trajectory_matches_oracle(hs_lib, oracle_lib, precision):
     fg, fo = g.fields(), o.fields()
     for k in fg:
         assert rel_l2(fg[k], fo[k]) < 1e-6, k
+
+
+@pytest.mark.parametrize("precision", [abi.HS_PREC_FP64, abi.HS_PREC_MIXED])
+def test_gmg_w_cycle_solution(hs_lib, oracle_lib, precision):
+    """The W-cycle option (mg_cycle = 2: each level below the finest visited twice, the
+    second visit starting from the first one's iterate, not from a zero guess) gives the
+    same p' solution as the oracle's tight CG, in no more FCG iterations than the V-cycle."""
+    case = configs.c2(scale=4)          # 64x16x16: levels 64x16x16, 32x8x8, 16x4x4
+    case.params.order = abi.HS_ORDER_RB
+    o = oracle_ctx(oracle_lib, case)
+    o.init()
+    for comp in range(3):
+        o.momentum(comp)
+    o.pressure_correction()
+    b = o.get("b")
+    op = abi.Params.from_buffer_copy(case.params)
+    op.p_tol, op.p_maxit = 1e-13, 20000
+    o2 = oracle_ctx(oracle_lib, case, op)
+    o2.init()
+    for comp in range(3):
+        o2.momentum(comp)
+    o2.pressure_correction()
+    xo, _ = o2.pressure_solve(b)
+    iters = {}
+    for cyc in (1, 2):
+        gp = abi.Params.from_buffer_copy(case.params)
+        gp.precond, gp.precision, gp.p_tol, gp.mg_cycle = abi.HS_PRECOND_GMG, precision, 1e-10, cyc
+        g = gpu_ctx(hs_lib, case, gp)
+        g.init()
+        for comp in range(3):
+            g.momentum(comp)
+        g.pressure_correction()
+        xg, sg = g.pressure_solve(b)
+        g.close()
+        assert sg["p_converged"] == 1 and sg["p_true_relres"] < 2e-10, (cyc, sg)
+        assert rel_l2(xg, xo) < 1e-7, cyc
+        iters[cyc] = sg["p_iters"]
+    assert iters[2] <= iters[1], iters
