@@ -364,6 +364,65 @@ __global__ __launch_bounds__(BT) void k_mg_restrict(Lv<T> F, Lv<T> C, int kc, in
     }
 }
 
+// The same restriction with the fine column streamed through registers: a thread walks
+// its fine column over the chunk's fine planes keeping the iterate of planes k-1, k, k+1
+// and cb of planes k, k+1, so every array line is read once; only in-plane neighbour
+// reads revisit lines, within the same plane.  TX x TY fine columns per workgroup (TY
+// even), the 2 x 2 in-plane sums of each coarse plane through LDS; the arithmetic per
+// fine cell is diag_of / offsum's, in their order.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_mg_restrict_s(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz,
+                                                           int zero_x, const SolveCtl* ctl) {
+    if (ctl->done) return;
+    __shared__ T red[TY][TX];
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i = 1 + tl.bx * TX + threadIdx.x, j = 1 + tl.by * TY + threadIdx.y;
+    const bool inside = i <= F.nx && j <= F.ny;
+    const int t = threadIdx.x + TX * threadIdx.y;
+    const int ci = t % (TX / 2), cj = t / (TX / 2);
+    const int I = 1 + tl.bx * (TX / 2) + ci, J = 1 + tl.by * (TY / 2) + cj;
+    const bool owner = t < (TX / 2) * (TY / 2) && I <= C.nx && J <= C.ny;
+    const int K0 = 1 + tl.bz * kc, K1 = min(C.nz, K0 + kc - 1);
+    const long sy = F.sy, sz = F.sz;
+    const long col = i + sy * j;
+    T xm = 0, x0 = 0, cb0 = 0;
+    if (inside) {
+        const long c = col + sz * (2 * K0 - 1);
+        xm = F.x[c - sz];
+        x0 = F.x[c];
+        cb0 = F.cb[c];
+    }
+    for (int K = K0; K <= K1; ++K) {
+        T r = 0;
+        for (int dk = 0; dk < 2; ++dk) {
+            const int k = 2 * K - 1 + dk;
+            if (k > F.nz) break;
+            const long f = col + sz * k;
+            T xp = 0, cb1 = 0;
+            if (inside) {
+                xp = F.x[f + sz];
+                cb1 = F.cb[f + sz];
+                const T cw = F.cw[f], ce = F.cw[f + 1], cs = F.cs[f], cn = F.cs[f + sy];
+                const T D = F.g[f] + cw + ce + cs + cn + cb0 + cb1;
+                const T off = cw * F.x[f - 1] + ce * F.x[f + 1] + cs * F.x[f - sy] + cn * F.x[f + sy] + cb0 * xm + cb1 * xp;
+                if (D > T(0)) r += F.b[f] - (D * x0 - off);
+            }
+            xm = x0;
+            x0 = xp;
+            cb0 = cb1;
+        }
+        red[threadIdx.y][threadIdx.x] = r;
+        __syncthreads();
+        if (owner) {
+            const T s = (red[2 * cj][2 * ci] + red[2 * cj][2 * ci + 1]) + (red[2 * cj + 1][2 * ci] + red[2 * cj + 1][2 * ci + 1]);
+            const long cc = I + C.sy * J + C.sz * K;
+            C.b[cc] = s;
+            if (zero_x) C.x[cc] = T(0);
+        }
+        __syncthreads();
+    }
+}
+
 // Trilinear interpolation of the coarse correction to fine cell (i,j,k) (1-based): the
 // fine centre sits a quarter coarse cell from its parent's centre, so each axis weighs
 // the parent 3/4 and the coarse neighbour on the fine cell's side 1/4 (27/64, 9/64,
@@ -412,6 +471,21 @@ inline TileShape tile_shape(int nx, int ny, int nz) {
 }
 
 // zero_x: also clear the coarse iterate (not needed when its first sweep reads it as zero)
+template <typename T, int TX, int TY>
+int launch_restrict_s(hs_ctx* c, const Lv<T>& F, const Lv<T>& C, bool zero_x) {
+    const int gx = (F.nx + TX - 1) / TX, gy = (F.ny + TY - 1) / TY;
+    // about 512K threads in flight, z chunks of coarse planes
+    long want = (1024L * 512 / (TX * TY)) / ((long)gx * gy);
+    if (want < 1) want = 1;
+    int kc = (int)((C.nz + want - 1) / want);
+    if (kc < 1) kc = 1;
+    const int gz = (C.nz + kc - 1) / kc;
+    k_mg_restrict_s<T, TX, TY><<<gx * gy * gz, dim3(TX, TY), 0, c->stream>>>(F, C, kc, gx, gy, gz, zero_x ? 1 : 0,
+                                                                          c->sctl);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
 template <typename T>
 int launch_restrict(hs_ctx* c, const Lv<T>& F, const Lv<T>& C, bool zero_x) {
     // fine-cell tiles in x and y, coarse planes in z
@@ -419,6 +493,9 @@ int launch_restrict(hs_ctx* c, const Lv<T>& F, const Lv<T>& C, bool zero_x) {
     TileShape s;
     s.kc = hsd::kchunk(C.nz, tiles);
     s.gr = dim3((F.nx + BX - 1) / BX, (F.ny + BY - 1) / BY, (C.nz + s.kc - 1) / s.kc);
+    // wide levels: the register-streamed form (256 x 4 or 128 x 8 fine columns)
+    if (F.nx > 128) return launch_restrict_s<T, 256, 4>(c, F, C, zero_x);
+    if (F.nx > 64) return launch_restrict_s<T, 128, 8>(c, F, C, zero_x);
     k_mg_restrict<T><<<hsd::grid1(s.gr), dim3(BX, BY), 0, c->stream>>>(F, C, s.kc, s.gr.x, s.gr.y, s.gr.z, zero_x ? 1 : 0,
                                                                       c->sctl);
     HS_CHECK(hipGetLastError());
