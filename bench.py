@@ -6,7 +6,8 @@ conductivity ratio 1000, lid + inlet forcing at Re = 1000, configuration physics
 red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual.
   value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG; the
                16^3 coarsest level solved with a dense FP32 inverse built on the matrix
-               cores plus FP64 refinement, rebuilt every --dense hierarchy builds)
+               cores plus FP64 refinement, rebuilt at most every --dense hierarchy builds and
+               sooner when the FCG iteration count rises by 2)
   fp64_value = the same step with the V-cycle in FP64
 Each timed step is one full SIMPLE outer iteration (x/y/z momentum, p' assembly +
 solve, velocity / pressure correction, conjugate energy sweep, set-for-next), all
@@ -126,9 +127,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 multigrid comparison run")
     ap.add_argument("--no-cg", action="store_true", help="skip the unpreconditioned FP64 CG drop-in step")
-    ap.add_argument("--dense", type=int, default=10, metavar="N",
+    ap.add_argument("--dense", type=int, default=50, metavar="N",
                     help="mixed path: coarsest level (16^3) solved with the dense FP32 inverse built on the "
-                         "matrix cores, rebuilt every N hierarchy builds (0: red-black sweeps there)")
+                         "matrix cores, rebuilt at most every N hierarchy builds and sooner when the FCG "
+                         "iteration count rises by 2 (0: red-black sweeps there)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
@@ -204,8 +206,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
                    "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG, V(2,1) on the finest level and V(2,2) below, fused RB-GS",
-                   "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt every {args.dense} builds, "
-                                    "2 FP64 refinement steps" if args.dense else "single-workgroup RB-GS tail"),
+                   "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
+                                    "(sooner on an FCG iteration rise of 2), 2 FP64 refinement steps"
+                                    if args.dense else "single-workgroup RB-GS tail"),
                    "coarse_fp64": "single-workgroup RB-GS tail",
                    "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},
         "fp64_value": round(fp64_value, 4) if fp64_value else None,

@@ -11,7 +11,8 @@
 //   A_kJ = P A_kJ,  A_Ik = -C_I,  A_kk = P
 // All products run on v_mfma_f32_16x16x4f32 (64x64 output tile per workgroup, four
 // wavefronts in a 2x2 arrangement of 32x32, K = 64 from LDS).  The inverse is kept
-// between hierarchy builds and refreshed every `mg_dense_coarse` builds.
+// between hierarchy builds and refreshed at most every `mg_dense_coarse` builds (sooner
+// when the flexible CG's iteration count shows the inverse has gone stale).
 //
 // Each V-cycle then solves the coarse system with the inverse and FP64 iterative
 // refinement against the level's own stencil:
@@ -354,11 +355,20 @@ int dense_setup(hs_ctx* c, const MGLevel* m, bool f32) {
     }
     DenseCoarse* D = c->dense;
     if (D->np != np) return HS_E_STATE;
+    // Refactor on the first build, once `mg_dense_coarse` builds have passed, or sooner
+    // when the last solve took 2 or more FCG iterations more than the first solve after
+    // the last factorization.  Between factorizations the FP64 refinement runs against
+    // the current level operator, so a stale inverse only weakens the coarse correction,
+    // and the iteration count shows when it does.
     const int every = std::max(1, c->prm.mg_dense_coarse);
-    if (D->builds % every == 0) {
+    const bool stale = D->iters_ref >= 0 && c->last_iters >= D->iters_ref + 2;
+    if (D->factorizations == 0 || D->since >= every || stale) {
         HS_TRY(f32 ? dense_factor_t<float>(c, m) : dense_factor_t<double>(c, m));
         D->factorizations += 1;
+        D->since = 0;
+        D->iters_ref = -1;
     }
+    D->since += 1;
     D->builds += 1;
     return HS_OK;
 }

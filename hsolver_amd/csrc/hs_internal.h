@@ -155,6 +155,8 @@ struct DenseCoarse {
     double *r = nullptr, *z = nullptr;   // right-hand side, solution (FP64)
     float* xf = nullptr;                 // FP32 input of the inverse apply
     long builds = 0, factorizations = 0;
+    long since = 0;                // hierarchy builds since the last factorization
+    int iters_ref = -1;            // FCG iterations of the first solve after it (-1: none yet)
 };
 
 struct hs_ctx {

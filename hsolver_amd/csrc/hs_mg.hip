@@ -875,6 +875,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
         batch = 2;
     }
     c->last_iters = h->iters;
+    if (c->dense && c->dense->iters_ref < 0) c->dense->iters_ref = h->iters;
     if (st) {
         st->p_iters = h->iters;
         st->p_converged = h->converged;

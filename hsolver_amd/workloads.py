@@ -118,7 +118,7 @@ def run_c5(iters, fd_cells=4):
            "iters": iters}
     p = abi.Params.from_buffer_copy(case.params)
     p.order, p.precond, p.precision = abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, abi.HS_PREC_MIXED
-    p.mg_dense_coarse = 10
+    p.mg_dense_coarse = 50
     ctx = core.create(case.grid, p, case.q)
     ctx.init()
     for w in range(2):

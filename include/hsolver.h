@@ -99,8 +99,10 @@ typedef struct {
                                          cell's aspect ratio exceeds 8, else V(6,6) everywhere */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
     int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense FP32
-                                         inverse built on the matrix cores, rebuilt every N hierarchy
-                                         builds (0: red-black sweeps on the coarsest level) */
+                                         inverse built on the matrix cores, rebuilt at most every N
+                                         hierarchy builds, sooner when a p' solve takes 2 or more FCG
+                                         iterations more than the first one after the last rebuild
+                                         (0: red-black sweeps on the coarsest level) */
     double  t_fluid_init;             /* fluid t_temp fill, 293 (fluid_solver.h:108) */
     double  t_solid_init;             /* solid t_temp fill, T1 = 333 (mobs_solver.h:55) */
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
