@@ -41,9 +41,13 @@ using mgl::lv;
 using mgl::diag_of;
 using mgl::offsum;
 
-// fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask
+// fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask,
+// and the level's active mask: the cell's east / north / top coefficients are the values
+// the neighbouring threads store (same faces, same cells), so D > 0 is decided here with
+// diag_of's operands in its order
 template <typename T>
-__global__ void k_mg_fine(Dims d, const uint8_t* cls, const double* ax, const double* ay, const double* az, Lv<T> L) {
+__global__ void k_mg_fine(Dims d, const uint8_t* cls, const double* ax, const double* ay, const double* az, Lv<T> L,
+                          uint8_t* act) {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
     if (i > d.nx) return;
     auto fl = [&](long ii, long jj, long kk) { return cls[idxP(d, ii, jj, kk)] == CELL_FLUID; };
@@ -52,9 +56,12 @@ __global__ void k_mg_fine(Dims d, const uint8_t* cls, const double* ax, const do
     const double w = ax[idxU(d, i, j, k)], e = ax[idxU(d, i + 1, j, k)];
     const double s = ay[idxV(d, i, j, k)], n = ay[idxV(d, i, j + 1, k)];
     const double b = az[idxW(d, i, j, k)], t = az[idxW(d, i, j, k + 1)];
-    L.cw[c] = (T)((f && fl(i - 1, j, k)) ? w : 0.0);
-    L.cs[c] = (T)((f && fl(i, j - 1, k)) ? s : 0.0);
-    L.cb[c] = (T)((f && fl(i, j, k - 1)) ? b : 0.0);
+    const T cw = (T)((f && fl(i - 1, j, k)) ? w : 0.0);
+    const T cs = (T)((f && fl(i, j - 1, k)) ? s : 0.0);
+    const T cb = (T)((f && fl(i, j, k - 1)) ? b : 0.0);
+    L.cw[c] = cw;
+    L.cs[c] = cs;
+    L.cb[c] = cb;
     double g = 0.0;
     if (f) {
         if (!fl(i - 1, j, k)) g += w;
@@ -64,7 +71,13 @@ __global__ void k_mg_fine(Dims d, const uint8_t* cls, const double* ax, const do
         if (!fl(i, j, k - 1)) g += b;
         if (!fl(i, j, k + 1)) g += t;
     }
-    L.g[c] = (T)g;
+    const T gT = (T)g;
+    L.g[c] = gT;
+    // the neighbours' west / south / bottom coefficients (their own f and the shared face)
+    const T ce = (T)((fl(i + 1, j, k) && f) ? e : 0.0);
+    const T cn = (T)((fl(i, j + 1, k) && f) ? n : 0.0);
+    const T ct = (T)((fl(i, j, k + 1) && f) ? t : 0.0);
+    act[c] = (gT + cw + ce + cs + cn + cb + ct) > T(0) ? 1 : 0;
 }
 
 template <typename T>
@@ -805,7 +818,8 @@ int build_hierarchy(hs_ctx* c) {
         }
     }
     Lv<T> L0 = lv<T>(c->mg[0]);
-    k_mg_fine<T><<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->ax, c->ay, c->az, L0);
+    k_mg_fine<T><<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->ax, c->ay, c->az, L0,
+                                                                          c->mg[0]->act);
     HS_CHECK(hipGetLastError());
     for (size_t l = 0; l + 1 < c->mg.size(); ++l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
@@ -818,6 +832,7 @@ int build_hierarchy(hs_ctx* c) {
         HS_CHECK(hipGetLastError());
     }
     for (MGLevel* m : c->mg) {
+        if (m->level == 0) continue;   // written by k_mg_fine
         k_mg_mask<T><<<dim3((m->nx + 63) / 64, m->ny, m->nz), 64, 0, c->stream>>>(lv<T>(m), m->act);
         HS_CHECK(hipGetLastError());
     }
