@@ -247,6 +247,153 @@ __global__ __launch_bounds__(BT) void k_e_smooth(EL<T> L, int tr, int kc, int co
     }
 }
 
+// One whole red-black sweep of the energy operator (A or A^T) in one pass, out of place
+// (x -> xn; the level's two iterate buffers swap afterwards): the two colour passes of
+// k_e_smooth, same operations in the same order, so the same values.  The pressure
+// smoother's scheme (k_mg_rbsweep): each thread owns a column of a TX x TY tile, at plane
+// step k loads its own faces of plane k+1 (x faces W/E, y faces S/N, the z face on top,
+// which is the next plane's bottom face; diffusion and flux), forms that plane's row once
+// and keeps it for the second colour there at step k+1.  The first colour of plane k+1 is
+// relaxed on the tile and a one-cell ring (ring rows by tile rows 0 and TY-1, side cells
+// by x threads 0 and 1, from memory); the second colour of plane k reads the first-colour
+// planes k-1, k, k+1 from LDS (four slots, one barrier per step).  Cells whose row has no
+// positive diagonal keep their value.  Byte offsets on the scalar base addresses.
+template <typename T, typename O>
+__device__ __forceinline__ T eld(const T* base, O off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
+template <typename T, bool TR>
+__device__ __forceinline__ Row<T> row_of(T dW, T dE, T dS, T dN, T dB, T dT, T fW, T fE, T fS, T fN, T fB, T fT) {
+    const T aw = dW + pos0(fW), ae = dE + pos0(-fE), as = dS + pos0(fS), an = dN + pos0(-fN);
+    const T ab = dB + pos0(fB), at = dT + pos0(-fT);
+    Row<T> r;
+    r.p = ae + aw + an + as + ab + at;
+    if (!TR) {
+        r.w = aw; r.e = ae; r.s = as; r.n = an; r.b = ab; r.t = at;
+    } else {
+        r.w = dW + pos0(-fW); r.e = dE + pos0(fE); r.s = dS + pos0(-fS);
+        r.n = dN + pos0(fN); r.b = dB + pos0(-fB); r.t = dT + pos0(fT);
+    }
+    return r;
+}
+
+// first-colour relaxation of one (ring) cell from memory
+template <typename T, typename O, bool TR>
+__device__ __forceinline__ T e_relax_at(const EL<T>& L, const T* xo, O c, O dy, O dz) {
+    constexpr O E = sizeof(T);
+    const Row<T> r = row_of<T, TR>(eld(L.dx, c), eld(L.dx, c + E), eld(L.dy, c), eld(L.dy, c + dy), eld(L.dz, c),
+                                   eld(L.dz, c + dz), eld(L.fx, c), eld(L.fx, c + E), eld(L.fy, c), eld(L.fy, c + dy),
+                                   eld(L.fz, c), eld(L.fz, c + dz));
+    if (!(r.p > T(0))) return eld(xo, c);
+    const T off = r.w * eld(xo, c - E) + r.e * eld(xo, c + E) + r.s * eld(xo, c - dy) + r.n * eld(xo, c + dy) +
+                  r.b * eld(xo, c - dz) + r.t * eld(xo, c + dz);
+    return (eld(L.b, c) + off) / r.p;
+}
+
+template <typename T, int TX, int TY, typename O, bool TR>
+__global__ __launch_bounds__(TX * TY) void k_e_rbsweep(EL<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1,
+                                                       int kc, int gx, int gy, int gz, const int* skip) {
+    constexpr int RX = TX + 2, RY = TY + 2;
+    constexpr O E = sizeof(T);
+    if (*skip) return;
+    __shared__ T X1[4][RY * RX];
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const int i0 = 1 + tl.bx * TX, j0 = 1 + tl.by * TY;
+    const int k0 = 1 + tl.bz * kc, k1 = min(L.nz, k0 + kc - 1);
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = i0 + lane, j = j0 + w;
+    const bool inside = i <= L.nx && j <= L.ny;
+    const int r = (w + 1) * RX + lane + 1;
+    const O dy = (O)L.sy * E, dz = (O)L.sz * E;
+    const O col = (O)(i + L.sy * j) * E;
+    const int rowring = (w == 0) ? j0 - 1 : (w == TY - 1 ? j0 + TY : -1);
+    const int rowslot = (w == 0) ? lane + 1 : (TY + 1) * RX + lane + 1;
+    const bool rowring_in = rowring >= 1 && rowring <= L.ny && i <= L.nx;
+    const int si = lane == 0 ? i0 - 1 : i0 + TX;
+    const int sslot = (w + 1) * RX + (lane == 0 ? 0 : TX + 1);
+    const bool side_in = lane < 2 && j <= L.ny && si >= 1 && si <= L.nx;
+
+    Row<T> R0{}, R1{};         // rows of planes k and k+1
+    T b0 = 0, b1 = 0;          // right-hand side, planes k and k+1
+    T dzB = 0, fzB = 0;        // bottom z faces of plane k+1 (the top faces of plane k)
+    T xk = 0, xk1 = 0, xk2 = 0;
+    T own1 = 0, own1n = 0;
+    const int kfirst = k0 - 2;
+    if (inside) {
+        if (kfirst >= 0) xk = eld(xo, col + (O)kfirst * dz);
+        xk1 = eld(xo, col + (O)(k0 - 1) * dz);
+        dzB = eld(L.dz, col + (O)(k0 - 1) * dz);
+        fzB = eld(L.fz, col + (O)(k0 - 1) * dz);
+    }
+    for (int k = kfirst; k <= k1; ++k) {
+        const int kp = k + 1;   // in [0, nz + 1]
+        if (inside) {
+            const O c = col + (O)kp * dz;
+            const T dzT = kp + 1 <= L.nz + 1 ? eld(L.dz, c + dz) : T(0);
+            const T fzT = kp + 1 <= L.nz + 1 ? eld(L.fz, c + dz) : T(0);
+            R1 = row_of<T, TR>(eld(L.dx, c), eld(L.dx, c + E), eld(L.dy, c), eld(L.dy, c + dy), dzB, dzT, eld(L.fx, c),
+                               eld(L.fx, c + E), eld(L.fy, c), eld(L.fy, c + dy), fzB, fzT);
+            b1 = eld(L.b, c);
+            if (kp + 1 <= L.nz + 1) xk2 = eld(xo, c + dz);
+            dzB = dzT;
+            fzB = fzT;
+        }
+        const bool kin = kp >= 1 && kp <= L.nz;
+        if (rowring >= 0) {
+            T v = T(0);
+            if (rowring_in && kin && ((i + rowring + kp) & 1) == c1)
+                v = e_relax_at<T, O, TR>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
+            X1[kp & 3][rowslot] = v;
+        }
+        if (lane < 2) {
+            T v = T(0);
+            if (side_in && kin && ((si + j + kp) & 1) == c1)
+                v = e_relax_at<T, O, TR>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
+            X1[kp & 3][sslot] = v;
+        }
+        {
+            T v = T(0);
+            if (inside && kin && ((i + j + kp) & 1) == c1) {
+                if (!(R1.p > T(0))) {
+                    v = xk1;
+                } else {
+                    const O c = col + (O)kp * dz;
+                    const T off = R1.w * eld(xo, c - E) + R1.e * eld(xo, c + E) + R1.s * eld(xo, c - dy) +
+                                  R1.n * eld(xo, c + dy) + R1.b * xk + R1.t * xk2;
+                    v = (b1 + off) / R1.p;
+                }
+            }
+            X1[kp & 3][r] = v;
+            own1n = v;
+        }
+        if (k >= k0) {
+            __syncthreads();
+            if (inside) {
+                T out;
+                if (((i + j + k) & 1) == c1) {
+                    out = own1;
+                } else if (!(R0.p > T(0))) {
+                    out = xk;
+                } else {
+                    const T* P0 = X1[k & 3];
+                    const T* Pm = X1[(k - 1) & 3];
+                    const T* Pp = X1[kp & 3];
+                    const T off = R0.w * P0[r - 1] + R0.e * P0[r + 1] + R0.s * P0[r - RX] + R0.n * P0[r + RX] +
+                                  R0.b * Pm[r] + R0.t * Pp[r];
+                    out = (b0 + off) / R0.p;
+                }
+                *reinterpret_cast<T*>(reinterpret_cast<char*>(xn) + col + (O)k * dz) = out;
+            }
+        }
+        R0 = R1;
+        b0 = b1;
+        xk = xk1;
+        xk1 = xk2;
+        own1 = own1n;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T e_residual_sum(const EL<T>& F, int I, int J, int K, int tr) {
     const int i0 = 2 * I - 1, j0 = 2 * J - 1, k0 = 2 * K - 1;
@@ -658,7 +805,7 @@ int e_alloc_levels(hs_ctx* c) {
         ELevel* m = new ELevel();
         m->nx = nx; m->ny = ny; m->nz = nz;
         m->sy = nx + 2; m->sz = (long)(nx + 2) * (ny + 2); m->n = m->sz * (nz + 2);
-        m->dx = m->dy = m->dz = m->fx = m->fy = m->fz = m->x = m->b = nullptr;
+        m->dx = m->dy = m->dz = m->fx = m->fy = m->fz = m->x = m->b = m->x2 = nullptr;
         m->own_faces = !(l == 0 && sizeof(T) == sizeof(double));
         E->lv.push_back(m);
         if (m->own_faces) {
@@ -668,6 +815,7 @@ int e_alloc_levels(hs_ctx* c) {
             m->dx = E->dx; m->dy = E->dy; m->dz = E->dz; m->fx = E->fx; m->fy = E->fy; m->fz = E->fz;
         }
         HS_TRY(e_dalloc(&m->x, m->n * sizeof(T)));
+        HS_TRY(e_dalloc(&m->x2, m->n * sizeof(T)));
         HS_TRY(e_dalloc(&m->b, m->n * sizeof(T)));
         if (std::min(nx, std::min(ny, nz)) <= 4) break;
         nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
@@ -716,12 +864,49 @@ int e_build(hs_ctx* c) {
     return HS_OK;
 }
 
+template <typename T, int TX, int TY>
+int launch_e_rbsweep(hs_ctx* c, const EL<T>& L, const T* src, T* dst, int c1, int tr, const int* skip) {
+    const int gx = (L.nx + TX - 1) / TX, gy = (L.ny + TY - 1) / TY;
+    long want = (1024L * 512 / (TX * TY)) / ((long)gx * gy);
+    if (want < 1) want = 1;
+    int kc = (int)((L.nz + want - 1) / want);
+    if (kc < 1) kc = 1;
+    const int gz = (L.nz + kc - 1) / kc;
+    const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
+    const dim3 g(gx * gy * gz), b(TX, TY);
+    if (small) {
+        if (tr) k_e_rbsweep<T, TX, TY, uint32_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+        else k_e_rbsweep<T, TX, TY, uint32_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+    } else {
+        if (tr) k_e_rbsweep<T, TX, TY, uint64_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+        else k_e_rbsweep<T, TX, TY, uint64_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+    }
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
 template <typename T>
-int e_smooth(hs_ctx* c, const ELevel* m, int tr, int sweeps, bool forward, const int* skip, bool level0) {
+int e_smooth(hs_ctx* c, ELevel* m, int tr, int sweeps, bool forward, const int* skip, bool level0) {
     EL<T> L = el<T>(m);
     const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
     const int kc = hsd::kchunk(L.nz, tiles);
     const dim3 gr = hsd::grid3(L.nx, L.ny, L.nz, kc);
+    // fused out-of-place sweeps (tile by level width), unless mg_sweep_rows < 0 asks for
+    // the two in-place colour passes; after each sweep the level's iterate buffers swap
+    if (c->prm.mg_sweep_rows >= 0) {
+        const int c1 = forward ? 0 : 1;
+        for (; sweeps > 0; --sweeps) {
+            const T* src = (const T*)m->x;
+            T* dst = (T*)m->x2;
+            const int slot = level0 ? prof_begin(c, HS_K_E_SMOOTH0) : -1;
+            if (L.nx > 128) HS_TRY((launch_e_rbsweep<T, 256, 4>(c, L, src, dst, c1, tr, skip)));
+            else if (L.nx > 64) HS_TRY((launch_e_rbsweep<T, 128, 8>(c, L, src, dst, c1, tr, skip)));
+            else HS_TRY((launch_e_rbsweep<T, 64, 8>(c, L, src, dst, c1, tr, skip)));
+            HS_TRY(prof_end(c, slot));
+            std::swap(m->x, m->x2);
+            L.x = (T*)m->x;
+        }
+    }
     for (int s = 0; s < sweeps; ++s)
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
@@ -759,7 +944,7 @@ int e_vcycle(hs_ctx* c, int tr) {
                                                      c->prm.mg_coarse_sweeps, skip);
     HS_CHECK(hipGetLastError());
     for (int l = t - 1; l >= 0; --l) {
-        const ELevel* f = E->lv[l];
+        ELevel* f = E->lv[l];
         const long tiles = (long)((f->nx + BX - 1) / BX) * ((f->ny + BY - 1) / BY);
         const int kc = hsd::kchunk(f->nz, tiles);
         const dim3 gr = hsd::grid3(f->nx, f->ny, f->nz, kc);
@@ -805,7 +990,9 @@ int e_bicgstab(hs_ctx* c, const double* b, int tr, double tol, int maxit, hs_sol
     HS_CHECK(hipMemcpyAsync(&E->ctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&E->ctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
     T* b0 = (T*)E->lv[0]->b;
-    T* x0 = (T*)E->lv[0]->x;
+    // the level-0 iterate (the V-cycle's output) moves between two buffers with the fused
+    // sweeps: read its current pointer at every use
+    auto x0 = [&]() { return (T*)E->lv[0]->x; };
     const int G = FLAT_BLOCKS;
     const EOp op = eop(E);
     k_bi_init<<<G, FLAT, 0, c->stream>>>(n, b, E->x, E->r, E->rh, E->p, E->v, c->partials);
@@ -816,16 +1003,16 @@ int e_bicgstab(hs_ctx* c, const double* b, int tr, double tol, int maxit, hs_sol
     while (true) {
         for (int it = 0; it < batch && launched < maxit; ++it, ++launched) {
             k_bi_restart<<<G, FLAT, 0, c->stream>>>(n, E->r, E->rh, E->ctl);
-            k_bi_p<T><<<G, FLAT, 0, c->stream>>>(n, E->r, E->p, E->v, b0, x0, E->ctl);
+            k_bi_p<T><<<G, FLAT, 0, c->stream>>>(n, E->r, E->p, E->v, b0, x0(), E->ctl);
             HS_TRY(e_vcycle<T>(c, tr));
-            k_bi_spmv<T><<<hsd::grid1(sh.gr), dim3(BX, BY), 0, c->stream>>>(d, op, tr, x0, E->ph, E->v, E->rh, 0, sh.kc,
+            k_bi_spmv<T><<<hsd::grid1(sh.gr), dim3(BX, BY), 0, c->stream>>>(d, op, tr, x0(), E->ph, E->v, E->rh, 0, sh.kc,
                                                                             sh.gr.x, sh.gr.y, sh.gr.z, c->partials,
                                                                             E->ctl);
             HS_TRY(bi_finish(c, BI_ALPHA, sh.nb));
-            k_bi_s<T><<<G, FLAT, 0, c->stream>>>(n, E->r, E->v, b0, x0, c->partials, E->ctl);
+            k_bi_s<T><<<G, FLAT, 0, c->stream>>>(n, E->r, E->v, b0, x0(), c->partials, E->ctl);
             HS_TRY(bi_finish(c, BI_S, G));
             HS_TRY(e_vcycle<T>(c, tr));
-            k_bi_spmv<T><<<hsd::grid1(sh.gr), dim3(BX, BY), 0, c->stream>>>(d, op, tr, x0, E->sh, E->t, E->r, 1, sh.kc,
+            k_bi_spmv<T><<<hsd::grid1(sh.gr), dim3(BX, BY), 0, c->stream>>>(d, op, tr, x0(), E->sh, E->t, E->r, 1, sh.kc,
                                                                             sh.gr.x, sh.gr.y, sh.gr.z, c->partials,
                                                                             E->ctl);
             HS_TRY(bi_finish(c, BI_OMEGA, sh.nb));
@@ -951,6 +1138,7 @@ void energy_free(hs_ctx* c) {
             for (void* a : f) if (a) hipFree(a);
         }
         if (m->x) hipFree(m->x);
+        if (m->x2) hipFree(m->x2);
         if (m->b) hipFree(m->b);
         delete m;
     }

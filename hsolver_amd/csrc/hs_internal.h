@@ -128,6 +128,7 @@ struct ELevel {
     int nx, ny, nz;
     long sy, sz, n;
     void *dx, *dy, *dz, *fx, *fy, *fz, *x, *b;
+    void* x2;          // second iterate buffer (the fused sweep runs out of place, then they swap)
     bool own_faces;    // false when the faces alias the FP64 fine operator
 };
 

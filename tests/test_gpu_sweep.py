@@ -51,3 +51,40 @@ def test_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, precision):
         assert it == it_ref, (rows, it, it_ref)
         for k in ref:
             assert np.array_equal(f[k], ref[k]), (rows, k)
+
+
+@pytest.mark.parametrize("precision", [abi.HS_PREC_MIXED, abi.HS_PREC_FP64], ids=["mixed", "fp64"])
+def test_fused_energy_sweep_bit_equal_to_colour_passes(hs_lib, precision):
+    """The energy multigrid's fused sweep against its two in-place colour passes, on one
+    frozen flow: the forward energy solve (A) and the adjoint solve (A^T) give the same
+    iteration counts and the same bits.  24^3 (tiles 64 x 8 with side cells in the
+    domain) and 150^3 (256-wide tiles on the finest level, partial tiles in y, 128 x 8
+    tiles on the next)."""
+    from helpers import cell_volumes, solid_pnorm
+    for case in (configs.c4(24), configs.c3(150)):
+        p = abi.Params.from_buffer_copy(case.params)
+        p.order, p.precond, p.precision = abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, precision
+        g = gpu_ctx(hs_lib, case, p)
+        g.init()
+        for _ in range(5):
+            g.step()
+        state = g.state()
+        g.close()
+        res = {}
+        for rows in (-1, 0):
+            p.mg_sweep_rows = rows
+            h = gpu_ctx(hs_lib, case, p)
+            h.init()
+            h.load_state(state)
+            fwd = h.energy_solve(tol=1e-12, maxit=400)
+            T = h.cells("tf")
+            solid = case.grid.cell_type.reshape(T.shape) == 1
+            _, dJdT = solid_pnorm(T, solid, cell_volumes(case))
+            dq, dk, adj = h.adjoint_energy(dJdT, tol=1e-12, maxit=400)
+            res[rows] = (fwd["iters"], adj["iters"], T, dq, dk)
+            h.close()
+        print(f"[energy sweep] {case.name} precision {precision}: forward {res[0][0]}, adjoint {res[0][1]} iterations",
+              flush=True)
+        assert res[0][0] == res[-1][0] and res[0][1] == res[-1][1], (res[0][:2], res[-1][:2])
+        for a, b in zip(res[0][2:], res[-1][2:]):
+            assert np.array_equal(a, b)
