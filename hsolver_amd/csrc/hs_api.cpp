@@ -285,6 +285,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_pre0 = -1;
     p->mg_post0 = -1;
     p->mg_cycle = 1;
+    p->mg_omega = 0.0f;  /* <= 0: chosen at hs_create from the grid's cell aspect ratio */
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
     p->t_wall = 293.0;
@@ -348,21 +349,23 @@ static double max_cell_aspect(const hs_ctx* c) {
     return r;
 }
 
-// Multigrid sweep counts left at -1 are chosen from the grid.  Near-isotropic cells:
-// V(2,2) with V(2,1) on the finest level (13 FCG iterations at 256^3, the fastest
-// measured).  Cells of aspect ratio above 8 couple far more strongly across their thin
-// side, which the point smoother has to carry: V(6,6) on every level (the 33.5M-cell
-// wall-stretched channel at aspect 50 takes 22 FCG iterations instead of 66, and its
-// SIMPLE iteration runs 35% faster).
+// Multigrid smoothing left unset (sweep counts -1, over-relaxation <= 0) is chosen from
+// the grid, as measured:
+//   near-isotropic cells: V(2,2) with V(2,1) on the finest level, omega 1.4 (the 256^3
+//     box: 11-12 FCG iterations against 13 at omega 1; above 1.5 it degrades);
+//   cells of aspect ratio above 8 couple far more strongly across their thin side, which
+//     the point smoother has to carry: V(6,6) on every level, omega 1.7 (the 33.5M-cell
+//     wall-stretched channel at aspect 50: 10 FCG iterations, against 22 at omega 1 and
+//     66 with the isotropic choice).
 static void resolve_sweeps(hs_ctx* c) {
     hs_params& p = c->prm;
-    if (p.mg_pre >= 0 && p.mg_post >= 0) return;
     const bool aniso = max_cell_aspect(c) > 8.0;
     if (p.mg_pre < 0) p.mg_pre = aniso ? 6 : 2;
     if (p.mg_post < 0) {
         p.mg_post = aniso ? 6 : 2;
         if (!aniso && p.mg_post0 < 0) p.mg_post0 = 1;
     }
+    if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.4f;
 }
 
 int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {

@@ -41,6 +41,14 @@ using mgl::lv;
 using mgl::diag_of;
 using mgl::offsum;
 
+// over-relaxed Gauss-Seidel value: x + omega (gs - x); omega == 1 gives gs itself (the
+// exact Gauss-Seidel expression, not x + 1 (gs - x))
+template <typename T>
+__device__ __forceinline__ T sor(T x, T gs, T omega) {
+    return omega == T(1) ? gs : x + omega * (gs - x);
+}
+
+
 // fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask,
 // and the level's active mask: the cell's east / north / top coefficients are the values
 // the neighbouring threads store (same faces, same cells), so D > 0 is decided here with
@@ -132,7 +140,7 @@ __global__ void k_mg_coarsen(Lv<T> F, Lv<T> C, T gscale) {
 
 // one colour of red-black Gauss-Seidel
 template <typename T>
-__global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, int gx, int gy, int gz,
+__global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, int gx, int gy, int gz, T omega,
                                                   const SolveCtl* ctl) {
     if (ctl->done) return;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
@@ -144,7 +152,7 @@ __global__ __launch_bounds__(BT) void k_mg_smooth(Lv<T> L, int kc, int colour, i
         const long c = i + L.sy * j + L.sz * k;
         const T D = diag_of(L, c);
         if (!(D > T(0))) continue;
-        L.x[c] = (L.b[c] + offsum(L, c)) / D;
+        L.x[c] = sor(L.x[c], (L.b[c] + offsum(L, c)) / D, omega);
     }
 }
 
@@ -185,8 +193,21 @@ __device__ __forceinline__ T ldx(const T* xo, O off) {
     return ZX ? T(0) : ldb(xo, off);
 }
 
+// an empty asm statement taking each value as an operand: the loads that produce them are
+// all issued before it, so the compiler cannot sink some of them under a later branch
+template <typename V>
+__device__ __forceinline__ int held(const V& v) {
+    asm volatile("" ::"v"(v));
+    return 0;
+}
+template <typename... V>
+__device__ __forceinline__ void issued_together(const V&... v) {
+    const int order[] = {held(v)...};
+    (void)order;
+}
+
 template <typename T, typename O, bool EAGER, bool ZX>
-__device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O dz) {
+__device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O dz, T omega) {
     constexpr O E = sizeof(T);
     const T g = ldb(L.g, c), cw = ldb(L.cw, c), ce = ldb(L.cw, c + E), cs = ldb(L.cs, c), cn = ldb(L.cs, c + dy),
             cb = ldb(L.cb, c), ct = ldb(L.cb, c + dz), b = ldb(L.b, c);
@@ -195,19 +216,25 @@ __device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O 
         const T x = ldx<T, O, ZX>(xo, c), xw = ldx<T, O, ZX>(xo, c - E), xe = ldx<T, O, ZX>(xo, c + E),
                 xs = ldx<T, O, ZX>(xo, c - dy), xn = ldx<T, O, ZX>(xo, c + dy), xb = ldx<T, O, ZX>(xo, c - dz),
                 xt = ldx<T, O, ZX>(xo, c + dz);
+        // one wait on memory: the iterate loads stay beside the operator loads instead of
+        // sinking under the D > 0 test into a second round trip
+        if (!ZX) issued_together(x, xw, xe, xs, xn, xb, xt);
         const T off = cw * xw + ce * xe + cs * xs + cn * xn + cb * xb + ct * xt;
         const T rel = (b + off) / D;
-        return (D > T(0)) ? rel : x;
+        return (D > T(0)) ? sor(x, rel, omega) : x;
     }
-    if (!(D > T(0))) return ldb(xo, c);
+    const T x = ldb(xo, c);
+    if (!(D > T(0))) return x;
     const T off = cw * ldb(xo, c - E) + ce * ldb(xo, c + E) + cs * ldb(xo, c - dy) + cn * ldb(xo, c + dy) +
                   cb * ldb(xo, c - dz) + ct * ldb(xo, c + dz);
-    return (b + off) / D;
+    return sor(x, (b + off) / D, omega);
 }
 
+// Float sweeps hold 8 waves per SIMD (at most 64 VGPRs): two 1024-thread workgroups per CU.
 template <typename T, int TX, int TY, typename O, bool EAGER, bool ZX>
-__global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1,
-                                                        int kc, int gx, int gy, int gz, const SolveCtl* ctl) {
+__global__ __launch_bounds__(TX * TY, sizeof(T) == sizeof(float) ? 8 : 1) void k_mg_rbsweep(
+    Lv<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1, int kc, int gx, int gy, int gz, T omega,
+    const SolveCtl* ctl) {
     constexpr int RX = TX + 2, RY = TY + 2;
     constexpr O E = sizeof(T);
     if (ctl->done) return;
@@ -260,13 +287,13 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
         if (rowring >= 0) {
             T v = T(0);
             if (rowring_in && kin && ((i + rowring + kp) & 1) == c1)
-                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
+                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz, omega);
             X1[kp & 3][rowslot] = v;
         }
         if (lane < 2) {
             T v = T(0);
             if (side_in && kin && ((si + j + kp) & 1) == c1)
-                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
+                v = relax_at<T, O, EAGER, ZX>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz, omega);
             X1[kp & 3][sslot] = v;
         }
         {
@@ -278,7 +305,7 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
             const T D = g1 + cw1 + ce1 + cs1 + cn1 + cbk1 + cbk2;
             const T off = cw1 * xw + ce1 * xe + cs1 * xs + cn1 * xnn + cbk1 * xk + cbk2 * xk2;
             const T rel = (b1 + off) / D;
-            const T v = on ? ((D > T(0)) ? rel : xk1) : T(0);
+            const T v = on ? ((D > T(0)) ? sor(xk1, rel, omega) : xk1) : T(0);
             X1[kp & 3][r] = v;
             own1n = v;
         }
@@ -293,7 +320,7 @@ __global__ __launch_bounds__(TX * TY) void k_mg_rbsweep(Lv<T> L, const T* __rest
                 const T off = cw0 * P0[r - 1] + ce0 * P0[r + 1] + cs0 * P0[r - RX] + cn0 * P0[r + RX] + cbk * Pm[r] +
                               cbk1 * Pp[r];
                 const T rel = (b0 + off) / D;
-                const T out = (((i + j + k) & 1) == c1) ? own1 : ((D > T(0)) ? rel : xk);
+                const T out = (((i + j + k) & 1) == c1) ? own1 : ((D > T(0)) ? sor(xk, rel, omega) : xk);
                 *reinterpret_cast<T*>(reinterpret_cast<char*>(xn) + col + (O)k * dz) = out;
             }
         }
@@ -320,12 +347,13 @@ int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bo
     // the profiled kernel is the full sweep (28 B per float cell); the zero-guess first
     // sweep of a cycle visit reads no iterate and is left out
     const int slot = (prof && !ZX) ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
+    const T omega = (T)c->prm.mg_omega;
     if (small)
         k_mg_rbsweep<T, TX, TY, uint32_t, EAGER, ZX>
-            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
+            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, c->sctl);
     else
         k_mg_rbsweep<T, TX, TY, uint64_t, EAGER, ZX>
-            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, c->sctl);
+            <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, c->sctl);
     HS_TRY(prof_end(c, slot));
     return HS_OK;
 }
@@ -724,7 +752,8 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
             const int slot = m->level == 0 ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
-            k_mg_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, kc, colour, gr.x, gr.y, gr.z, c->sctl);
+            k_mg_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, kc, colour, gr.x, gr.y, gr.z,
+                                                                        (T)c->prm.mg_omega, c->sctl);
             HS_TRY(prof_end(c, slot));
         }
     HS_CHECK(hipGetLastError());

@@ -114,7 +114,11 @@ typedef struct {
     int32_t mg_pre0, mg_post0;        /* sweeps on the finest level (-1: mg_pre / mg_post, or
                                          the by-grid choice above) */
     int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
-    int32_t reserved[2];
+    float   mg_omega;                 /* over-relaxation of the pressure multigrid's red-black
+                                         sweeps, x + omega (x_GS - x); 1 = Gauss-Seidel; <= 0
+                                         (default) = by grid: 1.4, or 1.7 when a cell's aspect
+                                         ratio exceeds 8 */
+    int32_t reserved[1];
 } hs_params;
 
 typedef struct {

@@ -25,8 +25,9 @@ CASES = {
 }
 
 
-def trajectory(hs_lib, case, precision, rows, steps=3):
+def trajectory(hs_lib, case, precision, rows, steps=3, omega=1.0):
     p = abi.Params.from_buffer_copy(case.params)
+    p.mg_omega = omega
     p.order = abi.HS_ORDER_RB
     p.precond = abi.HS_PRECOND_GMG
     p.precision = precision
@@ -48,6 +49,20 @@ def test_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, precision):
     ref, it_ref = trajectory(hs_lib, case, precision, -1)
     for rows in (0, 4, 8, 16):
         f, it = trajectory(hs_lib, case, precision, rows)
+        assert it == it_ref, (rows, it, it_ref)
+        for k in ref:
+            assert np.array_equal(f[k], ref[k]), (rows, k)
+
+
+@pytest.mark.parametrize("precision", [abi.HS_PREC_MIXED, abi.HS_PREC_FP64], ids=["mixed", "fp64"])
+@pytest.mark.parametrize("name", ["c3_90", "c2_64x16x16"])
+def test_over_relaxed_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, precision):
+    """The same with over-relaxed smoothing (mg_omega = 1.25): the fused sweep and the
+    colour passes apply x + omega (x_GS - x) alike."""
+    case = CASES[name]()
+    ref, it_ref = trajectory(hs_lib, case, precision, -1, omega=1.25)
+    for rows in (0, 8):
+        f, it = trajectory(hs_lib, case, precision, rows, omega=1.25)
         assert it == it_ref, (rows, it, it_ref)
         for k in ref:
             assert np.array_equal(f[k], ref[k]), (rows, k)
