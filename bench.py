@@ -207,7 +207,7 @@ def main() -> int:
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
                    "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG, V(2,1) on the finest level and V(2,2) below, fused red-black SOR (omega 1.4)",
                    "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
-                                    "(sooner on an FCG iteration rise of 2), 2 FP64 refinement steps"
+                                    "(sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
                                     if args.dense else "single-workgroup RB-GS tail"),
                    "coarse_fp64": "single-workgroup RB-GS tail",
                    "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},

@@ -280,12 +280,13 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_post = -1;
     p->mg_coarse_sweeps = 40;
     p->mg_dense_coarse = 0;
-    p->mg_dense_refine = 2;
+    p->mg_dense_refine = 1;
     p->mg_sweep_rows = 0;
     p->mg_pre0 = -1;
     p->mg_post0 = -1;
     p->mg_cycle = 1;
     p->mg_omega = 0.0f;  /* <= 0: chosen at hs_create from the grid's cell aspect ratio */
+    p->e_omega = 0.0f;   /* <= 0: 1 (Gauss-Seidel) at hs_create */
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
     p->t_wall = 293.0;
@@ -366,6 +367,12 @@ static void resolve_sweeps(hs_ctx* c) {
         if (!aniso && p.mg_post0 < 0) p.mg_post0 = 1;
     }
     if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.4f;
+    // energy multigrid (BiCGStab preconditioner): Gauss-Seidel.  The upwind convection
+    // operator is not symmetric, so over-relaxation has no convergence guarantee: on the
+    // aspect-50 channel omega 1.4 cut the energy and adjoint solves from 142 + 148 to
+    // 94 + 105 BiCGStab iterations on one frozen flow and left the adjoint diverging on a
+    // later one
+    if (!(p.e_omega > 0.0f)) p.e_omega = 1.0f;
 }
 
 int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {

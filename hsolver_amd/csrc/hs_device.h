@@ -102,6 +102,13 @@ __device__ __forceinline__ Tile tile_of(int gx, int gy, int gz) {
 }
 inline dim3 grid1(const dim3& g3) { return dim3(g3.x * g3.y * g3.z); }
 
+// over-relaxed Gauss-Seidel value x + omega (gs - x); omega == 1 gives gs itself (the
+// exact Gauss-Seidel expression, not x + 1 (gs - x))
+template <typename T>
+__device__ __forceinline__ T sor(T x, T gs, T omega) {
+    return omega == T(1) ? gs : x + omega * (gs - x);
+}
+
 inline int kchunk(int nk, long tiles) {
     long want = 4096 / (tiles > 0 ? tiles : 1);
     if (want < 1) want = 1;

@@ -232,7 +232,7 @@ __global__ void k_e_coarsen(EL<T> F, EFaces<T> C, int cnx, int cny, int cnz, lon
 // ------------------------------------------------------- V-cycle parts ---
 template <typename T>
 __global__ __launch_bounds__(BT) void k_e_smooth(EL<T> L, int tr, int kc, int colour, int gx, int gy, int gz,
-                                                 const int* skip) {
+                                                 T omega, const int* skip) {
     if (*skip) return;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
     const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(BT) void k_e_smooth(EL<T> L, int tr, int kc, int co
         const long c = i + L.sy * j + L.sz * k;
         const Row<T> r = lrow(L, c, tr);
         if (!(r.p > T(0))) continue;
-        L.x[c] = (L.b[c] + offs<T, T>(r, L.x, c, L.sy, L.sz)) / r.p;
+        L.x[c] = hsd::sor(L.x[c], (L.b[c] + offs<T, T>(r, L.x, c, L.sy, L.sz)) / r.p, omega);
     }
 }
 
@@ -280,7 +280,7 @@ __device__ __forceinline__ Row<T> row_of(T dW, T dE, T dS, T dN, T dB, T dT, T f
 
 // first-colour relaxation of one (ring) cell from memory
 template <typename T, typename O, bool TR>
-__device__ __forceinline__ T e_relax_at(const EL<T>& L, const T* xo, O c, O dy, O dz) {
+__device__ __forceinline__ T e_relax_at(const EL<T>& L, const T* xo, O c, O dy, O dz, T omega) {
     constexpr O E = sizeof(T);
     const Row<T> r = row_of<T, TR>(eld(L.dx, c), eld(L.dx, c + E), eld(L.dy, c), eld(L.dy, c + dy), eld(L.dz, c),
                                    eld(L.dz, c + dz), eld(L.fx, c), eld(L.fx, c + E), eld(L.fy, c), eld(L.fy, c + dy),
@@ -288,12 +288,12 @@ __device__ __forceinline__ T e_relax_at(const EL<T>& L, const T* xo, O c, O dy, 
     if (!(r.p > T(0))) return eld(xo, c);
     const T off = r.w * eld(xo, c - E) + r.e * eld(xo, c + E) + r.s * eld(xo, c - dy) + r.n * eld(xo, c + dy) +
                   r.b * eld(xo, c - dz) + r.t * eld(xo, c + dz);
-    return (eld(L.b, c) + off) / r.p;
+    return hsd::sor(eld(xo, c), (eld(L.b, c) + off) / r.p, omega);
 }
 
 template <typename T, int TX, int TY, typename O, bool TR>
 __global__ __launch_bounds__(TX * TY) void k_e_rbsweep(EL<T> L, const T* __restrict__ xo, T* __restrict__ xn, int c1,
-                                                       int kc, int gx, int gy, int gz, const int* skip) {
+                                                       int kc, int gx, int gy, int gz, T omega, const int* skip) {
     constexpr int RX = TX + 2, RY = TY + 2;
     constexpr O E = sizeof(T);
     if (*skip) return;
@@ -343,13 +343,13 @@ __global__ __launch_bounds__(TX * TY) void k_e_rbsweep(EL<T> L, const T* __restr
         if (rowring >= 0) {
             T v = T(0);
             if (rowring_in && kin && ((i + rowring + kp) & 1) == c1)
-                v = e_relax_at<T, O, TR>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz);
+                v = e_relax_at<T, O, TR>(L, xo, (O)(i + L.sy * rowring) * E + (O)kp * dz, dy, dz, omega);
             X1[kp & 3][rowslot] = v;
         }
         if (lane < 2) {
             T v = T(0);
             if (side_in && kin && ((si + j + kp) & 1) == c1)
-                v = e_relax_at<T, O, TR>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz);
+                v = e_relax_at<T, O, TR>(L, xo, (O)(si + L.sy * j) * E + (O)kp * dz, dy, dz, omega);
             X1[kp & 3][sslot] = v;
         }
         {
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(TX * TY) void k_e_rbsweep(EL<T> L, const T* __restr
                     const O c = col + (O)kp * dz;
                     const T off = R1.w * eld(xo, c - E) + R1.e * eld(xo, c + E) + R1.s * eld(xo, c - dy) +
                                   R1.n * eld(xo, c + dy) + R1.b * xk + R1.t * xk2;
-                    v = (b1 + off) / R1.p;
+                    v = hsd::sor(xk1, (b1 + off) / R1.p, omega);
                 }
             }
             X1[kp & 3][r] = v;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(TX * TY) void k_e_rbsweep(EL<T> L, const T* __restr
                     const T* Pp = X1[kp & 3];
                     const T off = R0.w * P0[r - 1] + R0.e * P0[r + 1] + R0.s * P0[r - RX] + R0.n * P0[r + RX] +
                                   R0.b * Pm[r] + R0.t * Pp[r];
-                    out = (b0 + off) / R0.p;
+                    out = hsd::sor(xk, (b0 + off) / R0.p, omega);
                 }
                 *reinterpret_cast<T*>(reinterpret_cast<char*>(xn) + col + (O)k * dz) = out;
             }
@@ -874,12 +874,13 @@ int launch_e_rbsweep(hs_ctx* c, const EL<T>& L, const T* src, T* dst, int c1, in
     const int gz = (L.nz + kc - 1) / kc;
     const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
     const dim3 g(gx * gy * gz), b(TX, TY);
+    const T omega = (T)c->prm.e_omega;
     if (small) {
-        if (tr) k_e_rbsweep<T, TX, TY, uint32_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
-        else k_e_rbsweep<T, TX, TY, uint32_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+        if (tr) k_e_rbsweep<T, TX, TY, uint32_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, skip);
+        else k_e_rbsweep<T, TX, TY, uint32_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, skip);
     } else {
-        if (tr) k_e_rbsweep<T, TX, TY, uint64_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
-        else k_e_rbsweep<T, TX, TY, uint64_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, skip);
+        if (tr) k_e_rbsweep<T, TX, TY, uint64_t, true><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, skip);
+        else k_e_rbsweep<T, TX, TY, uint64_t, false><<<g, b, 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, skip);
     }
     HS_CHECK(hipGetLastError());
     return HS_OK;
@@ -911,7 +912,7 @@ int e_smooth(hs_ctx* c, ELevel* m, int tr, int sweeps, bool forward, const int* 
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
             const int slot = level0 ? prof_begin(c, HS_K_E_SMOOTH0) : -1;
-            k_e_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, tr, kc, colour, gr.x, gr.y, gr.z, skip);
+            k_e_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, tr, kc, colour, gr.x, gr.y, gr.z, (T)c->prm.e_omega, skip);
             HS_TRY(prof_end(c, slot));
         }
     HS_CHECK(hipGetLastError());

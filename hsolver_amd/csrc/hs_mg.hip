@@ -41,12 +41,7 @@ using mgl::lv;
 using mgl::diag_of;
 using mgl::offsum;
 
-// over-relaxed Gauss-Seidel value: x + omega (gs - x); omega == 1 gives gs itself (the
-// exact Gauss-Seidel expression, not x + 1 (gs - x))
-template <typename T>
-__device__ __forceinline__ T sor(T x, T gs, T omega) {
-    return omega == T(1) ? gs : x + omega * (gs - x);
-}
+using hsd::sor;
 
 
 // fine level from the Mode C face coefficients (rho A^2/ap on U/V/W faces) + fluid mask,

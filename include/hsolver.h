@@ -107,7 +107,8 @@ typedef struct {
     double  t_solid_init;             /* solid t_temp fill, T1 = 333 (mobs_solver.h:55) */
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
-    int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve */
+    int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve
+                                         (default 1) */
     int32_t mg_sweep_rows;            /* tile of the fused red-black sweep: 0 = by level width
                                          (256x4, 128x8 or 64x8 columns); 4, 8, 16 = 64-wide tiles
                                          of that many rows; -1 = two in-place colour passes */
@@ -118,6 +119,10 @@ typedef struct {
                                          sweeps, x + omega (x_GS - x); 1 = Gauss-Seidel; <= 0
                                          (default) = by grid: 1.4, or 1.7 when a cell's aspect
                                          ratio exceeds 8 */
+    float   e_omega;                  /* the same for the energy multigrid (hs_energy_solve,
+                                         hs_energy_solve_rhs, hs_adjoint_energy); <= 0 (default)
+                                         = 1, Gauss-Seidel: the energy operator is not
+                                         symmetric and values above 1 can diverge */
     int32_t reserved[1];
 } hs_params;
 

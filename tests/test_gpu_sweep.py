@@ -68,17 +68,19 @@ def test_over_relaxed_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, preci
             assert np.array_equal(f[k], ref[k]), (rows, k)
 
 
+@pytest.mark.parametrize("e_omega", [1.0, 1.3], ids=["gauss_seidel", "over_relaxed"])
 @pytest.mark.parametrize("precision", [abi.HS_PREC_MIXED, abi.HS_PREC_FP64], ids=["mixed", "fp64"])
-def test_fused_energy_sweep_bit_equal_to_colour_passes(hs_lib, precision):
+def test_fused_energy_sweep_bit_equal_to_colour_passes(hs_lib, precision, e_omega):
     """The energy multigrid's fused sweep against its two in-place colour passes, on one
     frozen flow: the forward energy solve (A) and the adjoint solve (A^T) give the same
     iteration counts and the same bits.  24^3 (tiles 64 x 8 with side cells in the
     domain) and 150^3 (256-wide tiles on the finest level, partial tiles in y, 128 x 8
-    tiles on the next)."""
+    tiles on the next).  Gauss-Seidel sweeps and over-relaxed ones."""
     from helpers import cell_volumes, solid_pnorm
     for case in (configs.c4(24), configs.c3(150)):
         p = abi.Params.from_buffer_copy(case.params)
         p.order, p.precond, p.precision = abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, precision
+        p.e_omega = e_omega
         g = gpu_ctx(hs_lib, case, p)
         g.init()
         for _ in range(5):
