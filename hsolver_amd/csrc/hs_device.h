@@ -102,6 +102,19 @@ __device__ __forceinline__ Tile tile_of(int gx, int gy, int gz) {
 }
 inline dim3 grid1(const dim3& g3) { return dim3(g3.x * g3.y * g3.z); }
 
+// an empty asm statement taking each value as an operand: the loads that produce them are
+// all issued before it, so the compiler cannot sink some of them under a later branch
+template <typename V>
+__device__ __forceinline__ int held(const V& v) {
+    asm volatile("" ::"v"(v));
+    return 0;
+}
+template <typename... V>
+__device__ __forceinline__ void issued_together(const V&... v) {
+    const int order[] = {held(v)...};
+    (void)order;
+}
+
 // over-relaxed Gauss-Seidel value x + omega (gs - x); omega == 1 gives gs itself (the
 // exact Gauss-Seidel expression, not x + 1 (gs - x))
 template <typename T>

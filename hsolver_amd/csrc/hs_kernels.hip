@@ -29,6 +29,7 @@ using hsd::FLAT_BLOCKS;
 using hsd::block_sum;
 using hsd::put_partial;
 using hsd::wave_sum;
+using hsd::issued_together;
 
 __device__ __forceinline__ double posd(double a) { return (a < 0.0) ? 0.0 : a; }
 
@@ -212,68 +213,91 @@ struct MomArgs {
     int modeR;
 };
 
-__device__ __forceinline__ bool is_fluid(const MomArgs& a, long i, long j, long k) {
-    return a.cls[idxP(a.d, i, j, k)] == CELL_FLUID;
-}
-
 // One face of solveX/Y/Zmomentum (fluid_solver.h:596-625 / :513-542 / :427-457).
 // Returns 1 if the face value was updated, 2 if only its diagonal was written, 0 if
 // untouched; *dd receives the squared change.
 template <int COMP>
 __device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long k, double* dd) {
     const Dims& d = a.d;
+    // Every value the face can need is loaded before any is used, whatever the cell
+    // classes turn out to be: one wait on memory per face instead of four (the classes,
+    // the flux velocities, and the neighbours again after the a_P store, which the
+    // compiler cannot move across).  Every index of the face box is valid for all of them.
+    const long cN = COMP == 0 ? idxP(d, i - 1, j, k) : (COMP == 1 ? idxP(d, i, j - 1, k) : idxP(d, i, j, k - 1));
+    const int c1 = a.cls[idxP(d, i, j, k)], c0 = a.cls[cN];
+    const double xp = a.g.xp[i], xw = a.g.xw[i], xe = a.g.xe[i];
+    const double yp = a.g.yp[j], ys = a.g.ys[j], yn = a.g.yn[j];
+    const double zp = a.g.zp[k], zf = a.g.zf[k], zb = a.g.zb[k];
+    const double* U = a.u;
+    const double* V = a.v;
+    const double* W = a.w;
+    long s0, sy, sz;
+    double* X;
+    if (COMP == 0) { s0 = idxU(d, i, j, k); sy = d.sUy; sz = d.sUz; X = a.u; }
+    else if (COMP == 1) { s0 = idxV(d, i, j, k); sy = d.sVy; sz = d.sVz; X = a.v; }
+    else { s0 = idxW(d, i, j, k); sy = d.sPy; sz = d.sPz; X = a.w; }
+    // own component: the face and its six neighbours
+    const double x0 = X[s0], xW = X[s0 - 1], xE = X[s0 + 1], xS = X[s0 - sy], xN = X[s0 + sy], xB = X[s0 - sz],
+                 xT = X[s0 + sz];
+    // the other two components at the four corners each flux needs, and the pressures
+    double q0, q1, q2, q3, r0, r1, r2, r3, pm, pp;
+    if (COMP == 0) {
+        q0 = V[idxV(d, i, j, k)]; q1 = V[idxV(d, i - 1, j, k)]; q2 = V[idxV(d, i, j + 1, k)]; q3 = V[idxV(d, i - 1, j + 1, k)];
+        r0 = W[idxW(d, i, j, k + 1)]; r1 = W[idxW(d, i - 1, j, k + 1)]; r2 = W[idxW(d, i, j, k)]; r3 = W[idxW(d, i - 1, j, k)];
+        pm = a.pt[idxP(d, i - 1, j, k)];
+    } else if (COMP == 1) {
+        q0 = U[idxU(d, i, j, k)]; q1 = U[idxU(d, i, j - 1, k)]; q2 = U[idxU(d, i + 1, j, k)]; q3 = U[idxU(d, i + 1, j - 1, k)];
+        r0 = W[idxW(d, i, j, k + 1)]; r1 = W[idxW(d, i, j - 1, k + 1)]; r2 = W[idxW(d, i, j, k)]; r3 = W[idxW(d, i, j - 1, k)];
+        pm = a.pt[idxP(d, i, j - 1, k)];
+    } else {
+        q0 = U[idxU(d, i, j, k)]; q1 = U[idxU(d, i, j, k - 1)]; q2 = U[idxU(d, i + 1, j, k)]; q3 = U[idxU(d, i + 1, j, k - 1)];
+        r0 = V[idxV(d, i, j, k)]; r1 = V[idxV(d, i, j, k - 1)]; r2 = V[idxV(d, i, j + 1, k)]; r3 = V[idxV(d, i, j + 1, k - 1)];
+        pm = a.pt[idxP(d, i, j, k - 1)];
+    }
+    pp = a.pt[idxP(d, i, j, k)];
+    issued_together(c1, c0, xp, xw, xe, yp, ys, yn, zp, zf, zb, x0, xW, xE, xS, xN, xB, xT, q0, q1, q2, q3, r0, r1, r2, r3,
+                    pm, pp);
+
     bool active, touch;
     if (a.modeR) {
-        active = is_fluid(a, i, j, k);
+        active = c1 == CELL_FLUID;
         touch = false;
     } else {
-        const bool f1 = is_fluid(a, i, j, k);
-        const bool f0 = COMP == 0 ? is_fluid(a, i - 1, j, k) : (COMP == 1 ? is_fluid(a, i, j - 1, k) : is_fluid(a, i, j, k - 1));
+        const bool f1 = c1 == CELL_FLUID, f0 = c0 == CELL_FLUID;
         active = f0 && f1;
         touch = f0 || f1;
     }
     if (!active && !touch) return 0;
-    const double xp = a.g.xp[i], xw = a.g.xw[i], xe = a.g.xe[i];
-    const double yp = a.g.yp[j], ys = a.g.ys[j], yn = a.g.yn[j];
-    const double zp = a.g.zp[k], zf = a.g.zf[k], zb = a.g.zb[k];
     const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
     const double Dw = a.c2nu * Ayz / (xp + xw), De = a.c2nu * Ayz / (xp + xe);
     const double Ds = a.c2nu * Axz / (ys + yp), Dn = a.c2nu * Axz / (yn + yp);
     const double Db = a.c2nu * Axy / (zb + zp), Df = a.c2nu * Axy / (zf + zp);
-    const double* U = a.u;
-    const double* V = a.v;
-    const double* W = a.w;
     double Fmx, Fpx, Fmy, Fpy, Fmz, Fpz, b;
-    long s0, sy, sz;
-    double* X;
     const double c05 = a.c05;
     if (COMP == 0) {
-        s0 = idxU(d, i, j, k); sy = d.sUy; sz = d.sUz; X = a.u;
-        Fmx = c05 * Ayz * (U[s0] + U[s0 - 1]);
-        Fpx = c05 * Ayz * (U[s0] + U[s0 + 1]);
-        Fmy = c05 * Axz * (V[idxV(d, i, j, k)] + V[idxV(d, i - 1, j, k)]);
-        Fpy = c05 * Axz * (V[idxV(d, i, j + 1, k)] + V[idxV(d, i - 1, j + 1, k)]);
-        Fpz = c05 * Axy * (W[idxW(d, i, j, k + 1)] + W[idxW(d, i - 1, j, k + 1)]);
-        Fmz = c05 * Axy * (W[idxW(d, i, j, k)] + W[idxW(d, i - 1, j, k)]);
-        b = Ayz * (a.pt[idxP(d, i - 1, j, k)] - a.pt[idxP(d, i, j, k)]);
+        Fmx = c05 * Ayz * (x0 + xW);
+        Fpx = c05 * Ayz * (x0 + xE);
+        Fmy = c05 * Axz * (q0 + q1);
+        Fpy = c05 * Axz * (q2 + q3);
+        Fpz = c05 * Axy * (r0 + r1);
+        Fmz = c05 * Axy * (r2 + r3);
+        b = Ayz * (pm - pp);
     } else if (COMP == 1) {
-        s0 = idxV(d, i, j, k); sy = d.sVy; sz = d.sVz; X = a.v;
-        Fmy = c05 * Axz * (V[s0] + V[s0 - sy]);
-        Fpy = c05 * Axz * (V[s0] + V[s0 + sy]);
-        Fmx = c05 * Ayz * (U[idxU(d, i, j, k)] + U[idxU(d, i, j - 1, k)]);
-        Fpx = c05 * Ayz * (U[idxU(d, i + 1, j, k)] + U[idxU(d, i + 1, j - 1, k)]);
-        Fpz = c05 * Axy * (W[idxW(d, i, j, k + 1)] + W[idxW(d, i, j - 1, k + 1)]);
-        Fmz = c05 * Axy * (W[idxW(d, i, j, k)] + W[idxW(d, i, j - 1, k)]);
-        b = Axz * (a.pt[idxP(d, i, j - 1, k)] - a.pt[idxP(d, i, j, k)]);
+        Fmy = c05 * Axz * (x0 + xS);
+        Fpy = c05 * Axz * (x0 + xN);
+        Fmx = c05 * Ayz * (q0 + q1);
+        Fpx = c05 * Ayz * (q2 + q3);
+        Fpz = c05 * Axy * (r0 + r1);
+        Fmz = c05 * Axy * (r2 + r3);
+        b = Axz * (pm - pp);
     } else {
-        s0 = idxW(d, i, j, k); sy = d.sPy; sz = d.sPz; X = a.w;
-        Fmz = c05 * Axy * (W[s0] + W[s0 - sz]);
-        Fpz = c05 * Axy * (W[s0] + W[s0 + sz]);
-        Fmx = c05 * Ayz * (U[idxU(d, i, j, k)] + U[idxU(d, i, j, k - 1)]);
-        Fpx = c05 * Ayz * (U[idxU(d, i + 1, j, k)] + U[idxU(d, i + 1, j, k - 1)]);
-        Fmy = c05 * Axz * (V[idxV(d, i, j, k)] + V[idxV(d, i, j, k - 1)]);
-        Fpy = c05 * Axz * (V[idxV(d, i, j + 1, k)] + V[idxV(d, i, j + 1, k - 1)]);
-        b = Axy * (a.pt[idxP(d, i, j, k - 1)] - a.pt[idxP(d, i, j, k)]);
+        Fmz = c05 * Axy * (x0 + xB);
+        Fpz = c05 * Axy * (x0 + xT);
+        Fmx = c05 * Ayz * (q0 + q1);
+        Fpx = c05 * Ayz * (q2 + q3);
+        Fmy = c05 * Axz * (r0 + r1);
+        Fpy = c05 * Axz * (r2 + r3);
+        b = Axy * (pm - pp);
     }
     // reference upwind labels: y-momentum swaps x and y, z-momentum swaps z (Q7)
     const bool swx = a.modeR && COMP == 1, swy = a.modeR && COMP == 1, swz = a.modeR && COMP == 2;
@@ -286,12 +310,10 @@ __device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long
     const double ap = ae + aw + an + as + ab + af;
     a.ap[s0] = ap;
     if (!active) return 2;
-    const double nv = (ae * X[s0 + 1] + aw * X[s0 - 1] + an * X[s0 + sy] + as * X[s0 - sy] +
-                       af * X[s0 - sz] + ab * X[s0 + sz] + b) / ap;
-    const double old = X[s0];
-    const double val = a.relax * nv + (1 - a.relax) * old;
+    const double nv = (ae * xE + aw * xW + an * xN + as * xS + af * xB + ab * xT + b) / ap;
+    const double val = a.relax * nv + (1 - a.relax) * x0;
     X[s0] = val;
-    *dd = (val - old) * (val - old);
+    *dd = (val - x0) * (val - x0);
     return 1;
 }
 

@@ -188,18 +188,7 @@ __device__ __forceinline__ T ldx(const T* xo, O off) {
     return ZX ? T(0) : ldb(xo, off);
 }
 
-// an empty asm statement taking each value as an operand: the loads that produce them are
-// all issued before it, so the compiler cannot sink some of them under a later branch
-template <typename V>
-__device__ __forceinline__ int held(const V& v) {
-    asm volatile("" ::"v"(v));
-    return 0;
-}
-template <typename... V>
-__device__ __forceinline__ void issued_together(const V&... v) {
-    const int order[] = {held(v)...};
-    (void)order;
-}
+using hsd::issued_together;
 
 template <typename T, typename O, bool EAGER, bool ZX>
 __device__ __forceinline__ T relax_at(const Lv<T>& L, const T* xo, O c, O dy, O dz, T omega) {
