@@ -46,7 +46,8 @@ class Params(C.Structure):
                 ("t_fluid_init", C.c_double), ("t_solid_init", C.c_double), ("t_wall", C.c_double),
                 ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
-                ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("e_omega", C.c_float),
+                ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("mg_omega_coarse", C.c_float),
+                ("e_omega", C.c_float),
                 ("reserved", C.c_int32 * 1)]
 
 
@@ -116,6 +117,7 @@ def params_default(mode: int) -> Params:
     p.mg_post0 = -1
     p.mg_cycle = 1
     p.mg_omega = 0.0
+    p.mg_omega_coarse = 0.0
     p.e_omega = 0.0
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0

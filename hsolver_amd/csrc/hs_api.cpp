@@ -286,6 +286,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_post0 = -1;
     p->mg_cycle = 1;
     p->mg_omega = 0.0f;  /* <= 0: chosen at hs_create from the grid's cell aspect ratio */
+    p->mg_omega_coarse = 0.0f;
     p->e_omega = 0.0f;   /* <= 0: 1 (Gauss-Seidel) at hs_create */
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
@@ -352,10 +353,11 @@ static double max_cell_aspect(const hs_ctx* c) {
 
 // Multigrid smoothing left unset (sweep counts -1, over-relaxation <= 0) is chosen from
 // the grid, as measured:
-//   near-isotropic cells: V(2,2) with V(2,1) on the finest level, omega 1.4 (the 256^3
-//     box: 11-12 FCG iterations against 13 at omega 1; above 1.5 it degrades);
+//   near-isotropic cells: V(2,2) with V(2,1) on the finest level, omega 1.3 there and 1.6
+//     below (the 256^3 box: 11 FCG iterations in both precisions, against 13 at omega 1,
+//     12 at 1.4 on every level, 13-14 with 1.8 below);
 //   cells of aspect ratio above 8 couple far more strongly across their thin side, which
-//     the point smoother has to carry: V(6,6) on every level, omega 1.7 (the 33.5M-cell
+//     the point smoother has to carry: V(6,6) and omega 1.7 on every level (the 33.5M-cell
 //     wall-stretched channel at aspect 50: 10 FCG iterations, against 22 at omega 1 and
 //     66 with the isotropic choice).
 static void resolve_sweeps(hs_ctx* c) {
@@ -366,7 +368,8 @@ static void resolve_sweeps(hs_ctx* c) {
         p.mg_post = aniso ? 6 : 2;
         if (!aniso && p.mg_post0 < 0) p.mg_post0 = 1;
     }
-    if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.4f;
+    if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.3f;
+    if (!(p.mg_omega_coarse > 0.0f)) p.mg_omega_coarse = aniso ? 1.7f : 1.6f;
     // energy multigrid (BiCGStab preconditioner): Gauss-Seidel.  The upwind convection
     // operator is not symmetric, so over-relaxation has no convergence guarantee: on the
     // aspect-50 channel omega 1.4 cut the energy and adjoint solves from 142 + 148 to

@@ -317,7 +317,7 @@ __global__ __launch_bounds__(TX * TY, sizeof(T) == sizeof(float) ? 8 : 1) void k
 }
 
 template <typename T, int TX, int TY, bool ZX>
-int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof) {
+int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, T omega, bool prof) {
     constexpr bool EAGER = sizeof(T) == sizeof(float);
     const int gx = (L.nx + TX - 1) / TX, gy = (L.ny + TY - 1) / TY;
     // deep z chunks (a chunk reads two planes beyond its own): about 512K threads in
@@ -331,7 +331,6 @@ int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bo
     // the profiled kernel is the full sweep (28 B per float cell); the zero-guess first
     // sweep of a cycle visit reads no iterate and is left out
     const int slot = (prof && !ZX) ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
-    const T omega = (T)c->prm.mg_omega;
     if (small)
         k_mg_rbsweep<T, TX, TY, uint32_t, EAGER, ZX>
             <<<dim3(gx * gy * gz), dim3(TX, TY), 0, c->stream>>>(L, src, dst, c1, kc, gx, gy, gz, omega, c->sctl);
@@ -344,9 +343,9 @@ int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bo
 
 // zx: the iterate is zero (first sweep from a zero guess; the buffer is not read)
 template <typename T, int TX, int TY>
-int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, bool prof, bool zx) {
-    return zx ? launch_rbsweep_z<T, TX, TY, true>(c, L, src, dst, c1, prof)
-              : launch_rbsweep_z<T, TX, TY, false>(c, L, src, dst, c1, prof);
+int launch_rbsweep(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, T omega, bool prof, bool zx) {
+    return zx ? launch_rbsweep_z<T, TX, TY, true>(c, L, src, dst, c1, omega, prof)
+              : launch_rbsweep_z<T, TX, TY, false>(c, L, src, dst, c1, omega, prof);
 }
 
 // b_c = sum over active children of (b - A x); x_c = 0.
@@ -717,16 +716,18 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
     // 64 x 8 columns); 4, 8 or 16 force 64-wide tiles of that many rows; below 0 the two
     // in-place colour passes run instead
     const int rows = c->prm.mg_sweep_rows;
+    // over-relaxation: the finest level's, or that of the levels below it
+    const T omega = (T)(m->level == 0 ? c->prm.mg_omega : c->prm.mg_omega_coarse);
     if (sweeps > 0 && rows >= 0) {
         for (; sweeps > 0; --sweeps) {
             const T* src = (T*)m->x;
             T* dst = (T*)m->x2;
             const bool prof = m->level == 0;
-            if (rows == 4) HS_TRY((launch_rbsweep<T, 64, 4>(c, L, src, dst, c1, prof, zx)));
-            else if (rows == 16) HS_TRY((launch_rbsweep<T, 64, 16>(c, L, src, dst, c1, prof, zx)));
-            else if (rows > 0 || L.nx <= 64) HS_TRY((launch_rbsweep<T, 64, 8>(c, L, src, dst, c1, prof, zx)));
-            else if (L.nx <= 128) HS_TRY((launch_rbsweep<T, 128, 8>(c, L, src, dst, c1, prof, zx)));
-            else HS_TRY((launch_rbsweep<T, 256, 4>(c, L, src, dst, c1, prof, zx)));
+            if (rows == 4) HS_TRY((launch_rbsweep<T, 64, 4>(c, L, src, dst, c1, omega, prof, zx)));
+            else if (rows == 16) HS_TRY((launch_rbsweep<T, 64, 16>(c, L, src, dst, c1, omega, prof, zx)));
+            else if (rows > 0 || L.nx <= 64) HS_TRY((launch_rbsweep<T, 64, 8>(c, L, src, dst, c1, omega, prof, zx)));
+            else if (L.nx <= 128) HS_TRY((launch_rbsweep<T, 128, 8>(c, L, src, dst, c1, omega, prof, zx)));
+            else HS_TRY((launch_rbsweep<T, 256, 4>(c, L, src, dst, c1, omega, prof, zx)));
             zx = false;
             std::swap(m->x, m->x2);
             L.x = (T*)m->x;
@@ -736,8 +737,8 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
         for (int h = 0; h < 2; ++h) {
             const int colour = forward ? h : 1 - h;
             const int slot = m->level == 0 ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;
-            k_mg_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, kc, colour, gr.x, gr.y, gr.z,
-                                                                        (T)c->prm.mg_omega, c->sctl);
+            k_mg_smooth<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(L, kc, colour, gr.x, gr.y, gr.z, omega,
+                                                                        c->sctl);
             HS_TRY(prof_end(c, slot));
         }
     HS_CHECK(hipGetLastError());

@@ -116,9 +116,12 @@ typedef struct {
                                          the by-grid choice above) */
     int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
     float   mg_omega;                 /* over-relaxation of the pressure multigrid's red-black
-                                         sweeps, x + omega (x_GS - x); 1 = Gauss-Seidel; <= 0
-                                         (default) = by grid: 1.4, or 1.7 when a cell's aspect
-                                         ratio exceeds 8 */
+                                         sweeps on the finest level, x + omega (x_GS - x);
+                                         1 = Gauss-Seidel; <= 0 (default) = by grid: 1.3, or
+                                         1.7 when a cell's aspect ratio exceeds 8 */
+    float   mg_omega_coarse;          /* the same on the levels below the finest (down to the
+                                         single-workgroup tail, which stays at 1); <= 0
+                                         (default) = by grid: 1.6, or 1.7 as above */
     float   e_omega;                  /* the same for the energy multigrid (hs_energy_solve,
                                          hs_energy_solve_rhs, hs_adjoint_energy); <= 0 (default)
                                          = 1, Gauss-Seidel: the energy operator is not

@@ -25,9 +25,9 @@ CASES = {
 }
 
 
-def trajectory(hs_lib, case, precision, rows, steps=3, omega=1.0):
+def trajectory(hs_lib, case, precision, rows, steps=3, omega=(1.0, 1.0)):
     p = abi.Params.from_buffer_copy(case.params)
-    p.mg_omega = omega
+    p.mg_omega, p.mg_omega_coarse = omega
     p.order = abi.HS_ORDER_RB
     p.precond = abi.HS_PRECOND_GMG
     p.precision = precision
@@ -57,12 +57,12 @@ def test_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, precision):
 @pytest.mark.parametrize("precision", [abi.HS_PREC_MIXED, abi.HS_PREC_FP64], ids=["mixed", "fp64"])
 @pytest.mark.parametrize("name", ["c3_90", "c2_64x16x16"])
 def test_over_relaxed_fused_sweep_bit_equal_to_colour_passes(hs_lib, name, precision):
-    """The same with over-relaxed smoothing (mg_omega = 1.25): the fused sweep and the
-    colour passes apply x + omega (x_GS - x) alike."""
+    """The same with over-relaxed smoothing (1.25 on the finest level, 1.5 below): the
+    fused sweep and the colour passes apply x + omega (x_GS - x) alike, level by level."""
     case = CASES[name]()
-    ref, it_ref = trajectory(hs_lib, case, precision, -1, omega=1.25)
+    ref, it_ref = trajectory(hs_lib, case, precision, -1, omega=(1.25, 1.5))
     for rows in (0, 8):
-        f, it = trajectory(hs_lib, case, precision, rows, omega=1.25)
+        f, it = trajectory(hs_lib, case, precision, rows, omega=(1.25, 1.5))
         assert it == it_ref, (rows, it, it_ref)
         for k in ref:
             assert np.array_equal(f[k], ref[k]), (rows, k)
