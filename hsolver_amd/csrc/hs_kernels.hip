@@ -402,58 +402,92 @@ __device__ __forceinline__ double t_point_ref(const TArgs& a, long i, long j, lo
     return (conv + diff) / denom;
 }
 
-// Mode C conjugate energy coefficients (harmonic face conductance, upwind convection).
-__device__ __forceinline__ double t_coeffs_cfg(const TArgs& a, long i, long j, long k, double cf[6], double* bs) {
+// Mode C conjugate energy row at an interior cell (harmonic face conductance from the two
+// cells' half widths and conductivities, upwind convection from the face velocities; a face
+// on the domain boundary takes the cell's own width and conductivity), together with the
+// temperatures it couples.  Every value is loaded before any is used (one wait on memory
+// per cell): a neighbour outside the domain is loaded at the clamped index and replaced by
+// the cell's own values afterwards.
+struct TRowC {
+    int cls;
+    double ap, cf[6], bs;
+    double t0, tE, tW, tN, tS, tB, tT;
+};
+__device__ __forceinline__ TRowC t_row_cfg(const TArgs& a, long i, long j, long k) {
     const Dims& d = a.d;
+    const long c = idxP(d, i, j, k), cc = idxC(d, i - 1, j - 1, k - 1);
+    const double* T = a.t;
+    const long iw = i > 1 ? i - 1 : i, ie = i < d.nx ? i + 1 : i;
+    const long js = j > 1 ? j - 1 : j, jn = j < d.ny ? j + 1 : j;
+    const long kb = k > 1 ? k - 1 : k, kt = k < d.nz ? k + 1 : k;
+    TRowC r;
+    r.cls = a.cls[c];
     const double xp = a.g.xp[i], yp = a.g.yp[j], zp = a.g.zp[k];
+    const double gW = a.g.xp[i - 1], gE = a.g.xp[i + 1], gS = a.g.yp[j - 1], gN = a.g.yp[j + 1], gB = a.g.zp[k - 1],
+                 gT = a.g.zp[k + 1];
+    const double kP = a.kc[cc];
+    const double kW0 = a.kc[idxC(d, iw - 1, j - 1, k - 1)], kE0 = a.kc[idxC(d, ie - 1, j - 1, k - 1)];
+    const double kS0 = a.kc[idxC(d, i - 1, js - 1, k - 1)], kN0 = a.kc[idxC(d, i - 1, jn - 1, k - 1)];
+    const double kB0 = a.kc[idxC(d, i - 1, j - 1, kb - 1)], kT0 = a.kc[idxC(d, i - 1, j - 1, kt - 1)];
+    const double uW = a.u[idxU(d, i, j, k)], uE = a.u[idxU(d, i + 1, j, k)];
+    const double vS = a.v[idxV(d, i, j, k)], vN = a.v[idxV(d, i, j + 1, k)];
+    const double wB = a.w[idxW(d, i, j, k)], wT = a.w[idxW(d, i, j, k + 1)];
+    const double q = a.q[cc];
+    r.t0 = T[c]; r.tE = T[c + 1]; r.tW = T[c - 1]; r.tN = T[c + d.sPy]; r.tS = T[c - d.sPy];
+    r.tB = T[c - d.sPz]; r.tT = T[c + d.sPz];
+    issued_together(r.cls, xp, yp, zp, gW, gE, gS, gN, gB, gT, kP, kW0, kE0, kS0, kN0, kB0, kT0, uW, uE, vS, vN, wB, wT,
+                    q, r.t0, r.tE, r.tW, r.tN, r.tS, r.tB, r.tT);
     const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
-    const double kP = a.kc[idxC(d, i - 1, j - 1, k - 1)];
-    auto kof = [&](long ii, long jj, long kk) {
-        if (ii < 1 || ii > d.nx || jj < 1 || jj > d.ny || kk < 1 || kk > d.nz) return kP;
-        return a.kc[idxC(d, ii - 1, jj - 1, kk - 1)];
-    };
-    const double dW = (i - 1 < 1) ? xp : a.g.xp[i - 1], dE = (i + 1 > d.nx) ? xp : a.g.xp[i + 1];
-    const double dS = (j - 1 < 1) ? yp : a.g.yp[j - 1], dN = (j + 1 > d.ny) ? yp : a.g.yp[j + 1];
-    const double dB = (k - 1 < 1) ? zp : a.g.zp[k - 1], dT = (k + 1 > d.nz) ? zp : a.g.zp[k + 1];
-    const double DW = Ayz / (0.5 * xp / kP + 0.5 * dW / kof(i - 1, j, k));
-    const double DE = Ayz / (0.5 * xp / kP + 0.5 * dE / kof(i + 1, j, k));
-    const double DS = Axz / (0.5 * yp / kP + 0.5 * dS / kof(i, j - 1, k));
-    const double DN = Axz / (0.5 * yp / kP + 0.5 * dN / kof(i, j + 1, k));
-    const double DB = Axy / (0.5 * zp / kP + 0.5 * dB / kof(i, j, k - 1));
-    const double DT = Axy / (0.5 * zp / kP + 0.5 * dT / kof(i, j, k + 1));
+    const double kW = i - 1 < 1 ? kP : kW0, kE = i + 1 > d.nx ? kP : kE0;
+    const double kS = j - 1 < 1 ? kP : kS0, kN = j + 1 > d.ny ? kP : kN0;
+    const double kB = k - 1 < 1 ? kP : kB0, kT = k + 1 > d.nz ? kP : kT0;
+    const double dW = (i - 1 < 1) ? xp : gW, dE = (i + 1 > d.nx) ? xp : gE;
+    const double dS = (j - 1 < 1) ? yp : gS, dN = (j + 1 > d.ny) ? yp : gN;
+    const double dB = (k - 1 < 1) ? zp : gB, dT = (k + 1 > d.nz) ? zp : gT;
+    const double DW = Ayz / (0.5 * xp / kP + 0.5 * dW / kW);
+    const double DE = Ayz / (0.5 * xp / kP + 0.5 * dE / kE);
+    const double DS = Axz / (0.5 * yp / kP + 0.5 * dS / kS);
+    const double DN = Axz / (0.5 * yp / kP + 0.5 * dN / kN);
+    const double DB = Axy / (0.5 * zp / kP + 0.5 * dB / kB);
+    const double DT = Axy / (0.5 * zp / kP + 0.5 * dT / kT);
     const double rho = a.rho;
-    const double FW = rho * Ayz * a.u[idxU(d, i, j, k)], FE = rho * Ayz * a.u[idxU(d, i + 1, j, k)];
-    const double FS = rho * Axz * a.v[idxV(d, i, j, k)], FN = rho * Axz * a.v[idxV(d, i, j + 1, k)];
-    const double FB = rho * Axy * a.w[idxW(d, i, j, k)], FT = rho * Axy * a.w[idxW(d, i, j, k + 1)];
-    cf[0] = DE + posd(-FE);
-    cf[1] = DW + posd(FW);
-    cf[2] = DN + posd(-FN);
-    cf[3] = DS + posd(FS);
-    cf[4] = DB + posd(FB);
-    cf[5] = DT + posd(-FT);
-    *bs = a.q[idxC(d, i - 1, j - 1, k - 1)] * (xp * yp * zp);
-    return cf[0] + cf[1] + cf[2] + cf[3] + cf[4] + cf[5];
+    const double FW = rho * Ayz * uW, FE = rho * Ayz * uE;
+    const double FS = rho * Axz * vS, FN = rho * Axz * vN;
+    const double FB = rho * Axy * wB, FT = rho * Axy * wT;
+    r.cf[0] = DE + posd(-FE);
+    r.cf[1] = DW + posd(FW);
+    r.cf[2] = DN + posd(-FN);
+    r.cf[3] = DS + posd(FS);
+    r.cf[4] = DB + posd(FB);
+    r.cf[5] = DT + posd(-FT);
+    r.bs = q * (xp * yp * zp);
+    r.ap = r.cf[0] + r.cf[1] + r.cf[2] + r.cf[3] + r.cf[4] + r.cf[5];
+    return r;
 }
 
 __device__ __forceinline__ bool t_update(const TArgs& a, long i, long j, long k, double* dd) {
     const Dims& d = a.d;
     const long c = idxP(d, i, j, k);
-    const uint8_t cl = a.cls[c];
     double* T = a.t;
+    if (!a.solid && !a.modeR) {
+        const TRowC r = t_row_cfg(a, i, j, k);
+        if (r.cls == CELL_GHOST) return false;
+        const double nv = (r.cf[0] * r.tE + r.cf[1] * r.tW + r.cf[2] * r.tN + r.cf[3] * r.tS + r.cf[4] * r.tB +
+                           r.cf[5] * r.tT + r.bs) / r.ap;
+        const double val = nv * a.relax + (1 - a.relax) * r.t0;
+        T[c] = val;
+        *dd = (val - r.t0) * (val - r.t0);
+        return true;
+    }
+    const uint8_t cl = a.cls[c];
     double nv;
     if (a.solid) {
         if (cl != CELL_SOLID) return false;
         // MobSolver::solveTemperature mobs_solver.h:484-490 (Q12)
         nv = 1.0 / 6.0 * (T[c + 1] + T[c - 1] + T[c - d.sPy] + T[c + d.sPy] + T[c - d.sPz] + T[c + d.sPz]);
-    } else if (a.modeR) {
+    } else {   // Mode R fluid
         if (cl != CELL_FLUID) return false;
         nv = t_point_ref(a, i, j, k);
-    } else {
-        if (cl == CELL_GHOST) return false;
-        double cf[6], bs;
-        const double ap = t_coeffs_cfg(a, i, j, k, cf, &bs);
-        nv = (cf[0] * T[c + 1] + cf[1] * T[c - 1] + cf[2] * T[c + d.sPy] + cf[3] * T[c - d.sPy] +
-              cf[4] * T[c - d.sPz] + cf[5] * T[c + d.sPz] + bs) / ap;
     }
     const double old = T[c];
     const double val = nv * a.relax + (1 - a.relax) * old;
@@ -846,13 +880,10 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
     if (i <= d.nx && j <= d.ny) {
         const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
         for (int k = k0; k <= k1; ++k) {
-            const long c = idxP(d, i, j, k);
-            if (a.cls[c] == CELL_GHOST) continue;
-            double cf[6], bs;
-            const double ap = t_coeffs_cfg(a, i, j, k, cf, &bs);
-            const double* T = a.t;
-            acc += ap * T[c] - (cf[0] * T[c + 1] + cf[1] * T[c - 1] + cf[2] * T[c + d.sPy] + cf[3] * T[c - d.sPy] +
-                                cf[4] * T[c - d.sPz] + cf[5] * T[c + d.sPz]) - bs;
+            const TRowC r = t_row_cfg(a, i, j, k);
+            if (r.cls == CELL_GHOST) continue;
+            acc += r.ap * r.t0 - (r.cf[0] * r.tE + r.cf[1] * r.tW + r.cf[2] * r.tN + r.cf[3] * r.tS + r.cf[4] * r.tB +
+                                  r.cf[5] * r.tT) - r.bs;
         }
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
