@@ -45,7 +45,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0):
+def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0, fields_out=None):
     from hsolver_amd import abi, core
     lib = core.library()
     lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
@@ -73,8 +73,20 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
         lib.hs_prof_get(ctx.h, C.byref(n), C.byref(ms))
         prof = (n.value, ms.value)
         lib.hs_prof_set(ctx.h, 0)
+    if fields_out is not None:
+        fields_out.update(ctx.fields())      # after the timed region
     ctx.close()
     return (t1 - t0), stats, prof
+
+
+def field_distance(a, b):
+    import numpy as np
+    out = {}
+    for k in ("u", "v", "w", "p", "t"):
+        if k in a and k in b:
+            den = float(np.linalg.norm(b[k]))
+            out[k] = float(np.linalg.norm(a[k] - b[k]) / den) if den > 0 else float(np.linalg.norm(a[k]))
+    return out
 
 
 def pmc_traffic(n):
@@ -157,15 +169,17 @@ def main() -> int:
     log(f"workload {case.workload}: {ncell} cells, solid fraction {solid_frac:.3f} "
         f"(built in {time.perf_counter() - t_case:.1f} s)")
 
+    f_mixed, f_fp64 = {}, {}
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                       prof_kernel=1, dense=args.dense)
+                                       prof_kernel=1, dense=args.dense, fields_out=f_mixed)
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
     fp64_value = None
     st_fp64 = []
     if not args.no_fp64:
-        dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup)
+        dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup,
+                                   fields_out=f_fp64)
         fp64_value = args.steps / dt64
         log(f"fp64 gmg: {fp64_value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_fp64]}")
 
@@ -213,6 +227,9 @@ def main() -> int:
                    "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},
         "fp64_value": round(fp64_value, 4) if fp64_value else None,
         "mixed_over_fp64": round(value / fp64_value, 4) if fp64_value else None,
+        # both paths run the same warmup + timed SIMPLE iterations from the same start, each
+        # solving p' to 1e-8: relative L2 distance of the mixed path's fields from the FP64 one
+        "mixed_vs_fp64_rel_l2": field_distance(f_mixed, f_fp64) if f_fp64 else None,
         "p_iters_mixed": [s["p_iters"] for s in st_mixed],
         "p_iters_fp64": [s["p_iters"] for s in st_fp64],
         "p_true_relres_max": max(relres) if relres else None,
