@@ -424,7 +424,15 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     else { A(&c->ax, d.nU); A(&c->ay, d.nV); A(&c->az, d.nW); }
     A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->kp2, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
     A(&c->dres, std::max(std::max(d.nU, d.nV), std::max(d.nW, d.nP)));
-    A(&c->partials, 65536);
+    {
+        // block partials: two per block of the largest z-marching grid (64 x 4 column tiles,
+        // z chunks as the launchers choose them), and at least 64K
+        const long tiles = (long)((d.nx + 1 + 63) / 64) * ((d.ny + 1 + 3) / 4);
+        long want = 4096 / tiles;
+        if (want < 1) want = 1;
+        const long kc = (d.nz + 1 + want - 1) / want, gz = (d.nz + 1 + kc - 1) / kc;
+        A(&c->partials, std::max(65536L, 2 * tiles * gz + 1024));
+    }
     if (e == HS_OK) e = dalloc(&c->cls, (size_t)d.nP);
     if (e == HS_OK) e = dalloc(&c->sctl, 1);
     if (e == HS_OK) e = dalloc(&c->wctl, 1);

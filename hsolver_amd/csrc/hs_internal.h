@@ -235,6 +235,7 @@ enum FinishKind : int {
     FIN_FCG_START,      // <r,z> -> gamma
     FIN_FCG_UPDATE,     // <r,r> -> stop rule
     FIN_FCG_BETA,       // <z,r>, <z,q> -> beta, gamma
+    FIN_DOT2,           // two plain sums -> out[0], out[1]
 };
 
 // ---- kernel launchers (defined in the .hip files) ----

@@ -757,7 +757,7 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
     const int tid = threadIdx.x;
     const double a = hsd::sum_partials<1024>(partials, n, tid);
     double b = 0.0;
-    if (kind == FIN_FCG_BETA) b = hsd::sum_partials<1024>(partials + n, n, tid);
+    if (kind == FIN_FCG_BETA || kind == FIN_DOT2) b = hsd::sum_partials<1024>(partials + n, n, tid);
     if (tid != 0) return;
     switch (kind) {
         case FIN_SWEEP: sweep_control(wctl, a); break;
@@ -780,6 +780,7 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
             ctl->beta = ctl->gamma / ctl->gamma_old;
             break;
         case FIN_DOT: *out = a; break;
+        case FIN_DOT2: out[0] = a; out[1] = b; break;
         case FIN_FCG_SETUP:
             ctl->bb = a; ctl->rr = a; ctl->iters = 0;
             ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
@@ -808,6 +809,30 @@ __global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* 
     for (int k = k0; k <= k1; ++k) y[idxP(d, i, j, k)] = 0.0;
     spmv_column(
         o, [x](long q) { return x[q]; }, i, j, k0, k1, [&](long c, double v, double) { y[c] = v; }, [](long, double) {});
+}
+
+// ||b - A x||^2 and ||b||^2 in one pass over the fluid rows (block partials, two per
+// block).  b is zero outside fluid cells (the assembled right-hand side, or a caller's
+// vector after k_mask_fluid), and so are the rows of A there, so the other cells add nothing.
+__global__ __launch_bounds__(BT) void k_resid_norms(OpArgs o, const double* x, const double* b, int kc, int gx, int gy,
+                                                    int gz, double* partials) {
+    const Dims& d = o.d;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    double rr = 0.0, bb = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
+        spmv_column(
+            o, [x](long q) { return x[q]; }, i, j, k0, k1,
+            [&](long c, double y, double) {
+                const double bc = b[c], r = bc - y;
+                rr += r * r;
+                bb += bc * bc;
+            },
+            [](long, double) {});
+    }
+    hsd::put_partial2<BT>(rr, bb, tid, blockIdx.x, gx * gy * gz, partials);
 }
 
 // zero entries outside fluid cells
@@ -1185,12 +1210,18 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
 }
 
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm) {
-    HS_TRY(spmv(c, x, c->kq));
-    k_axpby<<<FLAT_BLOCKS, 256, 0, c->stream>>>(c->d.nP, 1.0, b, -1.0, c->kq, c->kq);
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    const int nb = (int)(gr.x * gr.y * gr.z);
+    k_resid_norms<<<nb, dim3(BX, BY), 0, c->stream>>>(op_args(c), x, b, kc, gr.x, gr.y, gr.z, c->partials);
     HS_CHECK(hipGetLastError());
-    double rr, bb;
-    HS_TRY(dot(c, c->kq, c->kq, &rr));
-    HS_TRY(dot(c, b, b, &bb));
+    HS_TRY(launch_finish(c, FIN_DOT2, nb, &c->sctl->red[0]));
+    double h2[2];
+    HS_CHECK(hipMemcpyAsync(h2, &c->sctl->red[0], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipStreamSynchronize(c->stream));
+    const double rr = h2[0], bb = h2[1];
     *relres = bb > 0 ? sqrt(rr / bb) : 0.0;
     *bnorm = sqrt(bb);
     return HS_OK;
