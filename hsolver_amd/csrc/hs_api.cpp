@@ -353,9 +353,10 @@ static double max_cell_aspect(const hs_ctx* c) {
 
 // Multigrid smoothing left unset (sweep counts -1, over-relaxation <= 0) is chosen from
 // the grid, as measured:
-//   near-isotropic cells: V(2,2) with V(2,1) on the finest level, omega 1.3 there and 1.6
-//     below (the 256^3 box: 11 FCG iterations in both precisions, against 13 at omega 1,
-//     12 at 1.4 on every level, 13-14 with 1.8 below);
+//   near-isotropic cells: one pre- and two post-sweeps on the finest level, V(3,3) below,
+//     omega 1.3 on the finest level and 1.6 below (the 256^3 box: 9 FCG iterations in both
+//     precisions; V(2,1) on the finest level with V(2,2) below takes 11, the same at
+//     omega 1 takes 13, and 1.5 or 1.7 below the finest level take 10);
 //   cells of aspect ratio above 8 couple far more strongly across their thin side, which
 //     the point smoother has to carry: V(6,6) and omega 1.7 on every level (the 33.5M-cell
 //     wall-stretched channel at aspect 50: 10 FCG iterations, against 22 at omega 1 and
@@ -363,10 +364,13 @@ static double max_cell_aspect(const hs_ctx* c) {
 static void resolve_sweeps(hs_ctx* c) {
     hs_params& p = c->prm;
     const bool aniso = max_cell_aspect(c) > 8.0;
-    if (p.mg_pre < 0) p.mg_pre = aniso ? 6 : 2;
+    if (p.mg_pre < 0) {
+        p.mg_pre = aniso ? 6 : 3;
+        if (!aniso && p.mg_pre0 < 0) p.mg_pre0 = 1;
+    }
     if (p.mg_post < 0) {
-        p.mg_post = aniso ? 6 : 2;
-        if (!aniso && p.mg_post0 < 0) p.mg_post0 = 1;
+        p.mg_post = aniso ? 6 : 3;
+        if (!aniso && p.mg_post0 < 0) p.mg_post0 = 2;
     }
     if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.3f;
     if (!(p.mg_omega_coarse > 0.0f)) p.mg_omega_coarse = aniso ? 1.7f : 1.6f;

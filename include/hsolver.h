@@ -95,7 +95,7 @@ typedef struct {
     int32_t precision;                /* HS_PREC_* */
     int32_t mg_levels;                /* multigrid levels including the finest; 0 = down to <= 4 cells a side */
     int32_t mg_pre, mg_post;          /* RB-GS sweeps before / after coarse correction; -1 (default)
-                                         = by grid: V(2,2) and V(2,1) on the finest level when no
+                                         = by grid: V(3,3) and V(1,2) on the finest level when no
                                          cell's aspect ratio exceeds 8, else V(6,6) everywhere */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
     int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense FP32
