@@ -219,7 +219,7 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
-                   "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.3 on the finest level, 1.6 below)",
+                   "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.2 on the finest level, 1.6 below)",
                    "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
                                     "(sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
                                     if args.dense else "single-workgroup RB-GS tail"),

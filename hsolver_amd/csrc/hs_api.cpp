@@ -354,9 +354,10 @@ static double max_cell_aspect(const hs_ctx* c) {
 // Multigrid smoothing left unset (sweep counts -1, over-relaxation <= 0) is chosen from
 // the grid, as measured:
 //   near-isotropic cells: one pre- and two post-sweeps on the finest level, V(3,3) below,
-//     omega 1.3 on the finest level and 1.6 below (the 256^3 box: 9 FCG iterations in both
-//     precisions; V(2,1) on the finest level with V(2,2) below takes 11, the same at
-//     omega 1 takes 13, and 1.5 or 1.7 below the finest level take 10);
+//     omega 1.2 on the finest level and 1.6 below (the 256^3 box: 9 FCG iterations in both
+//     precisions on every SIMPLE step; 1.3 on the finest level takes 9-10, V(2,1) on the
+//     finest level with V(2,2) below takes 11, the same at omega 1 takes 13, and 1.5 or
+//     1.7 below the finest level take 10);
 //   cells of aspect ratio above 8 couple far more strongly across their thin side, which
 //     the point smoother has to carry: V(6,6) and omega 1.7 on every level (the 33.5M-cell
 //     wall-stretched channel at aspect 50: 10 FCG iterations, against 22 at omega 1 and
@@ -372,7 +373,7 @@ static void resolve_sweeps(hs_ctx* c) {
         p.mg_post = aniso ? 6 : 3;
         if (!aniso && p.mg_post0 < 0) p.mg_post0 = 2;
     }
-    if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.3f;
+    if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.2f;
     if (!(p.mg_omega_coarse > 0.0f)) p.mg_omega_coarse = aniso ? 1.7f : 1.6f;
     // energy multigrid (BiCGStab preconditioner): Gauss-Seidel.  The upwind convection
     // operator is not symmetric, so over-relaxation has no convergence guarantee: on the

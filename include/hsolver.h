@@ -117,7 +117,7 @@ typedef struct {
     int32_t mg_cycle;                 /* visits of each level below the finest per cycle: 1 V, 2 W */
     float   mg_omega;                 /* over-relaxation of the pressure multigrid's red-black
                                          sweeps on the finest level, x + omega (x_GS - x);
-                                         1 = Gauss-Seidel; <= 0 (default) = by grid: 1.3, or
+                                         1 = Gauss-Seidel; <= 0 (default) = by grid: 1.2, or
                                          1.7 when a cell's aspect ratio exceeds 8 */
     float   mg_omega_coarse;          /* the same on the levels below the finest (down to the
                                          single-workgroup tail, which stays at 1); <= 0
