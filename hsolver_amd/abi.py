@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
                 ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("mg_omega_coarse", C.c_float),
-                ("e_omega", C.c_float),
+                ("e_omega", C.c_float), ("mg_wide_offsets", C.c_int32),
                 ("reserved", C.c_int32 * 1)]
 
 

@@ -872,7 +872,7 @@ int launch_e_rbsweep(hs_ctx* c, const EL<T>& L, const T* src, T* dst, int c1, in
     int kc = (int)((L.nz + want - 1) / want);
     if (kc < 1) kc = 1;
     const int gz = (L.nz + kc - 1) / kc;
-    const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
+    const bool small = !c->prm.mg_wide_offsets && (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
     const dim3 g(gx * gy * gz), b(TX, TY);
     const T omega = (T)c->prm.e_omega;
     if (small) {

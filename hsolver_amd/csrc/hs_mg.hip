@@ -327,7 +327,7 @@ int launch_rbsweep_z(hs_ctx* c, const Lv<T>& L, const T* src, T* dst, int c1, T 
     int kc = (int)((L.nz + want - 1) / want);
     if (kc < 1) kc = 1;
     const int gz = (L.nz + kc - 1) / kc;
-    const bool small = (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
+    const bool small = !c->prm.mg_wide_offsets && (double)L.sz * (L.nz + 2) * sizeof(T) < 2147483648.0;
     // the profiled kernel is the full sweep (28 B per float cell); the zero-guess first
     // sweep of a cycle visit reads no iterate and is left out
     const int slot = (prof && !ZX) ? prof_begin(c, HS_K_MG_SMOOTH0) : -1;

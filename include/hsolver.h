@@ -126,6 +126,9 @@ typedef struct {
                                          hs_energy_solve_rhs, hs_adjoint_energy); <= 0 (default)
                                          = 1, Gauss-Seidel: the energy operator is not
                                          symmetric and values above 1 can diverge */
+    int32_t mg_wide_offsets;          /* 1 = the fused multigrid sweeps address every level with
+                                         64-bit byte offsets, the form used for level arrays of
+                                         2 GiB and more (0, default: 32-bit below 2 GiB) */
     int32_t reserved[1];
 } hs_params;
 

@@ -25,9 +25,10 @@ CASES = {
 }
 
 
-def trajectory(hs_lib, case, precision, rows, steps=3, omega=(1.0, 1.0)):
+def trajectory(hs_lib, case, precision, rows, steps=3, omega=(1.0, 1.0), wide=0):
     p = abi.Params.from_buffer_copy(case.params)
     p.mg_omega, p.mg_omega_coarse = omega
+    p.mg_wide_offsets = wide
     p.order = abi.HS_ORDER_RB
     p.precond = abi.HS_PRECOND_GMG
     p.precision = precision
@@ -105,3 +106,43 @@ def test_fused_energy_sweep_bit_equal_to_colour_passes(hs_lib, precision, e_omeg
         assert res[0][0] == res[-1][0] and res[0][1] == res[-1][1], (res[0][:2], res[-1][:2])
         for a, b in zip(res[0][2:], res[-1][2:]):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [abi.HS_PREC_MIXED, abi.HS_PREC_FP64], ids=["mixed", "fp64"])
+def test_wide_offset_sweeps_bit_equal(hs_lib, precision):
+    """The 64-bit-offset form of the fused sweeps, which levels of 2 GiB and more take,
+    forced on small grids against the 32-bit form: the same SIMPLE trajectories (pressure
+    multigrid, default over-relaxation) and the same energy and adjoint solves, bit for bit."""
+    from helpers import cell_volumes, solid_pnorm
+    case = CASES["c3_90"]()
+    ref, it_ref = trajectory(hs_lib, case, precision, 0, omega=(1.25, 1.5))
+    f, it = trajectory(hs_lib, case, precision, 0, omega=(1.25, 1.5), wide=1)
+    print(f"[sweep] wide offsets, precision {precision}: FCG iterations {it} against {it_ref}", flush=True)
+    assert it == it_ref
+    for k in ref:
+        assert np.array_equal(f[k], ref[k]), k
+    ecase = configs.c4(24)
+    p = abi.Params.from_buffer_copy(ecase.params)
+    p.order, p.precond, p.precision = abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, precision
+    g = gpu_ctx(hs_lib, ecase, p)
+    g.init()
+    for _ in range(5):
+        g.step()
+    state = g.state()
+    g.close()
+    res = {}
+    for wide in (0, 1):
+        p.mg_wide_offsets = wide
+        h = gpu_ctx(hs_lib, ecase, p)
+        h.init()
+        h.load_state(state)
+        fwd = h.energy_solve(tol=1e-12, maxit=400)
+        T = h.cells("tf")
+        solid = ecase.grid.cell_type.reshape(T.shape) == 1
+        _, dJdT = solid_pnorm(T, solid, cell_volumes(ecase))
+        dq, dk, adj = h.adjoint_energy(dJdT, tol=1e-12, maxit=400)
+        res[wide] = (fwd["iters"], adj["iters"], T, dq, dk)
+        h.close()
+    assert res[0][:2] == res[1][:2], (res[0][:2], res[1][:2])
+    for a, b in zip(res[0][2:], res[1][2:]):
+        assert np.array_equal(a, b)
