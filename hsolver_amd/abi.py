@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("fail_on_noconv", C.c_int32), ("mg_dense_refine", C.c_int32),
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
                 ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("mg_omega_coarse", C.c_float),
-                ("e_omega", C.c_float), ("mg_wide_offsets", C.c_int32),
+                ("e_omega", C.c_float), ("mg_wide_offsets", C.c_int32), ("e_pre", C.c_int32), ("e_post", C.c_int32),
                 ("reserved", C.c_int32 * 1)]
 
 
@@ -118,6 +118,7 @@ def params_default(mode: int) -> Params:
     p.mg_cycle = 1
     p.mg_omega = 0.0
     p.mg_omega_coarse = 0.0
+    p.e_pre = p.e_post = -1
     p.e_omega = 0.0
     p.t_fluid_init = 293.0
     p.t_solid_init = 333.0

@@ -287,6 +287,7 @@ void hs_params_default(int32_t mode, hs_params* p) {
     p->mg_cycle = 1;
     p->mg_omega = 0.0f;  /* <= 0: chosen at hs_create from the grid's cell aspect ratio */
     p->mg_omega_coarse = 0.0f;
+    p->e_pre = p->e_post = -1;   /* by grid at hs_create */
     p->e_omega = 0.0f;   /* <= 0: 1 (Gauss-Seidel) at hs_create */
     p->t_fluid_init = 293.0;
     p->t_solid_init = 333.0;
@@ -375,6 +376,12 @@ static void resolve_sweeps(hs_ctx* c) {
     }
     if (!(p.mg_omega > 0.0f)) p.mg_omega = aniso ? 1.7f : 1.2f;
     if (!(p.mg_omega_coarse > 0.0f)) p.mg_omega_coarse = aniso ? 1.7f : 1.6f;
+    // energy multigrid sweeps: the pressure cycle's counts.  On the aspect-50 channel V(8,8)
+    // took the energy and adjoint solves of one frozen flow from 6.0 to 5.1 s (176 + 187 to
+    // 125 + 119 BiCGStab iterations) and those of a later flow from 5.27 to 5.33 s (165 + 157
+    // to 136 + 122), so the counts stay shared unless set
+    if (p.e_pre < 0) p.e_pre = p.mg_pre;
+    if (p.e_post < 0) p.e_post = p.mg_post;
     // energy multigrid (BiCGStab preconditioner): Gauss-Seidel.  The upwind convection
     // operator is not symmetric, so over-relaxation has no convergence guarantee: on the
     // aspect-50 channel omega 1.4 cut the energy and adjoint solves from 142 + 148 to

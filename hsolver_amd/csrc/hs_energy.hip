@@ -929,7 +929,7 @@ int e_vcycle(hs_ctx* c, int tr) {
     while (t < nl && (long)E->lv[t]->nx * E->lv[t]->ny * E->lv[t]->nz > E_TAIL_MAX_CELLS) ++t;
     if (t >= nl) t = nl - 1;
     for (int l = 0; l < t; ++l) {
-        HS_TRY(e_smooth<T>(c, E->lv[l], tr, c->prm.mg_pre, true, skip, l == 0));
+        HS_TRY(e_smooth<T>(c, E->lv[l], tr, c->prm.e_pre, true, skip, l == 0));
         const ELevel* m = E->lv[l + 1];
         const ELevel* f = E->lv[l];
         const long tiles = (long)((f->nx + BX - 1) / BX) * ((f->ny + BY - 1) / BY);
@@ -941,7 +941,7 @@ int e_vcycle(hs_ctx* c, int tr) {
     }
     ELPack<T> P;
     for (int l = 0; l < nl && l < E_MAX_LEVELS; ++l) P.l[l] = el<T>(E->lv[l]);
-    k_e_tail<T><<<1, E_TAIL_THREADS, 0, c->stream>>>(P, t, nl, tr, c->prm.mg_pre, c->prm.mg_post,
+    k_e_tail<T><<<1, E_TAIL_THREADS, 0, c->stream>>>(P, t, nl, tr, c->prm.e_pre, c->prm.e_post,
                                                      c->prm.mg_coarse_sweeps, skip);
     HS_CHECK(hipGetLastError());
     for (int l = t - 1; l >= 0; --l) {
@@ -952,7 +952,7 @@ int e_vcycle(hs_ctx* c, int tr) {
         k_e_prolong<T><<<hsd::grid1(gr), dim3(BX, BY), 0, c->stream>>>(el<T>(f), el<T>(E->lv[l + 1]), kc, gr.x, gr.y,
                                                                        gr.z, skip);
         HS_CHECK(hipGetLastError());
-        HS_TRY(e_smooth<T>(c, f, tr, c->prm.mg_post, false, skip, l == 0));
+        HS_TRY(e_smooth<T>(c, f, tr, c->prm.e_post, false, skip, l == 0));
     }
     return HS_OK;
 }

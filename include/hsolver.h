@@ -129,6 +129,8 @@ typedef struct {
     int32_t mg_wide_offsets;          /* 1 = the fused multigrid sweeps address every level with
                                          64-bit byte offsets, the form used for level arrays of
                                          2 GiB and more (0, default: 32-bit below 2 GiB) */
+    int32_t e_pre, e_post;            /* sweeps of the energy multigrid before / after its coarse
+                                         correction; -1 (default) = mg_pre / mg_post */
     int32_t reserved[1];
 } hs_params;
 
