@@ -203,15 +203,14 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
     HS_TRY(assemble_rhs(c));
     hs_stats loc{};
     int e;
-    if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc);
-    else e = cg_solve(c, c->b, c->kx, &loc);
+    // the true residual of the solution is formed before the solver's last readback
+    if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc, true);
+    else e = cg_solve(c, c->b, c->kx, &loc, true);
     if (e != HS_OK) return e;
     HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
-    double rel, bn;
-    HS_TRY(true_relres(c, c->b, c->kx, &rel, &bn));
     if (st) {
         st->p_iters = loc.p_iters; st->p_converged = loc.p_converged; st->p_relres = loc.p_relres;
-        st->p_true_relres = rel; st->mass_imbalance = bn;
+        st->p_true_relres = loc.p_true_relres; st->mass_imbalance = loc.mass_imbalance;
     }
     if (c->prm.fail_on_noconv && !loc.p_converged) return HS_E_NOCONV;
     return HS_OK;
@@ -584,10 +583,14 @@ int hs_fluid_solve(hs_ctx* c, hs_stats* st) {
     HS_TRY(do_correct(c));
     HS_TRY(do_energy(c, &loc.t_sweeps, &loc.t_residual));
     if (!modeR(c)) {
-        double heat = 0, res = 0;
-        HS_TRY(energy_balance(c, &heat, &res));
-        loc.heat_in = heat;
-        loc.energy_residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
+        if (c->defer_balance) {
+            HS_TRY(energy_balance_async(c));   // hs_step reads it after its last synchronisation
+        } else {
+            double heat = 0, res = 0;
+            HS_TRY(energy_balance(c, &heat, &res));
+            loc.heat_in = heat;
+            loc.energy_residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
+        }
     }
     if (st) *st = loc;
     return HS_OK;
@@ -631,10 +634,18 @@ int hs_set_for_next(hs_ctx* c) {
 
 int hs_step(hs_ctx* c, hs_stats* st) {
     hs_stats loc{};
-    HS_TRY(hs_fluid_solve(c, &loc));
+    c->defer_balance = true;
+    const int e = hs_fluid_solve(c, &loc);
+    c->defer_balance = false;
+    HS_TRY(e);
     HS_TRY(hs_solid_solve(c, &loc));
     HS_TRY(hs_exchange(c));
-    HS_TRY(hs_set_for_next(c));
+    HS_TRY(hs_set_for_next(c));   // synchronises: the deferred energy balance is in h_sctl
+    if (!modeR(c)) {
+        const double res = c->h_sctl->red[1], heat = c->h_sctl->red[2];
+        loc.heat_in = heat;
+        loc.energy_residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
+    }
     if (st) *st = loc;
     return HS_OK;
 }

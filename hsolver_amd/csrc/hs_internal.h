@@ -167,6 +167,7 @@ struct hs_ctx {
     int device = 0;
     bool inited = false;
     bool have_matrix = false;
+    bool defer_balance = false;  // hs_step reads the energy balance at its final synchronisation
 
     // host copies
     std::vector<double> X, Y, Z;
@@ -256,19 +257,28 @@ int assemble_matrix(hs_ctx* c);
 int assemble_rhs(hs_ctx* c);
 int spmv(hs_ctx* c, const double* x, double* y);
 int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
-int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
+// with_resid: the true residual ||b - Ax|| / ||b|| and ||b|| (p_true_relres, mass_imbalance)
+// are formed on the device before each readback of the solve's control block and read back
+// with it, so a converged solve returns them without another synchronisation
+int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid = false);
 int cg_spmv_dot(hs_ctx* c, const double* p, double* q);
 int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* pn, double* q);
 int mask_fluid(hs_ctx* c, double* x);
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);
+// queues ||b - Ax||^2 and ||b||^2 into sctl->red[0], red[1] (no synchronisation)
+int launch_resid_norms(hs_ctx* c, const double* b, const double* x);
+void resid_from_red(const SolveCtl* h, hs_stats* st);
 int correct_velocity(hs_ctx* c);
 int update_pressure(hs_ctx* c);
 int energy_sweep(hs_ctx* c);
 int solid_sweep(hs_ctx* c);
 int exchange_kernel(hs_ctx* c);
 int energy_balance(hs_ctx* c, double* heat, double* resid);
+// the same sums queued with an asynchronous copy into h_sctl->red[1] (residual) and red[2]
+// (heat input), read after the caller's next synchronisation
+int energy_balance_async(hs_ctx* c);
 int gmg_setup(hs_ctx* c);
-int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st);
+int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid = false);
 void gmg_free(hs_ctx* c);
 // dense coarsest-level solve (hs_dense.hip)
 int dense_max_cells();

@@ -1166,7 +1166,7 @@ int dot(hs_ctx* c, const double* a, const double* b, double* out_host) {
 // HYPRE ParCSR PCG with the identity preconditioner (fluid_solver.h:1789-1790, settings
 // :1871-1875), x0 = 0.  Iterations are launched in batches; the stop rule is evaluated
 // on the device and later launches of a finished solve return at entry.
-int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     const Dims& d = c->d;
     const long n = d.nP;
     SolveCtl* h = c->h_sctl;
@@ -1194,6 +1194,7 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
             k_cg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, c->kr, c->kp, c->sctl);
         }
         HS_CHECK(hipGetLastError());
+        if (with_resid) HS_TRY(launch_resid_norms(c, b, x));
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
         HS_CHECK(hipStreamSynchronize(c->stream));
         if (h->done || launched >= c->prm.p_maxit) break;
@@ -1204,12 +1205,13 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
         st->p_iters = h->iters;
         st->p_converged = h->converged;
         st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
+        if (with_resid) resid_from_red(h, st);
     }
     if (!std::isfinite(h->rr)) return HS_E_NAN;
     return HS_OK;
 }
 
-int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm) {
+int launch_resid_norms(hs_ctx* c, const double* b, const double* x) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
@@ -1217,7 +1219,17 @@ int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, dou
     const int nb = (int)(gr.x * gr.y * gr.z);
     k_resid_norms<<<nb, dim3(BX, BY), 0, c->stream>>>(op_args(c), x, b, kc, gr.x, gr.y, gr.z, c->partials);
     HS_CHECK(hipGetLastError());
-    HS_TRY(launch_finish(c, FIN_DOT2, nb, &c->sctl->red[0]));
+    return launch_finish(c, FIN_DOT2, nb, &c->sctl->red[0]);
+}
+
+void resid_from_red(const SolveCtl* h, hs_stats* st) {
+    const double rr = h->red[0], bb = h->red[1];
+    st->p_true_relres = bb > 0 ? sqrt(rr / bb) : 0.0;
+    st->mass_imbalance = sqrt(bb);
+}
+
+int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm) {
+    HS_TRY(launch_resid_norms(c, b, x));
     double h2[2];
     HS_CHECK(hipMemcpyAsync(h2, &c->sctl->red[0], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
@@ -1243,7 +1255,7 @@ int update_pressure(hs_ctx* c) {
     return launch_fill(c, c->pprime, d.nP, 0.0);
 }
 
-int energy_balance(hs_ctx* c, double* heat, double* resid) {
+static int energy_balance_launch(hs_ctx* c) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
@@ -1253,7 +1265,18 @@ int energy_balance(hs_ctx* c, double* heat, double* resid) {
     HS_TRY(launch_finish(c, FIN_DOT, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
     k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials);
     HS_CHECK(hipGetLastError());
-    HS_TRY(launch_finish(c, FIN_DOT, FLAT_BLOCKS, &c->sctl->red[2]));
+    return launch_finish(c, FIN_DOT, FLAT_BLOCKS, &c->sctl->red[2]);
+}
+
+int energy_balance_async(hs_ctx* c) {
+    HS_TRY(energy_balance_launch(c));
+    HS_CHECK(hipMemcpyAsync(&c->h_sctl->red[1], &c->sctl->red[1], 2 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    return HS_OK;
+}
+
+int energy_balance(hs_ctx* c, double* heat, double* resid) {
+    HS_TRY(energy_balance_launch(c));
     double h2[2];
     HS_CHECK(hipMemcpyAsync(h2, &c->sctl->red[1], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));

@@ -856,7 +856,7 @@ int build_hierarchy(hs_ctx* c) {
 }
 
 template <typename T>
-int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     const long n = c->d.nP;
     HS_TRY(build_hierarchy<T>(c));
     SolveCtl* h = c->h_sctl;
@@ -898,6 +898,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
             HS_TRY(launch_finish(c, FIN_FCG_BETA, G, nullptr));
         }
         HS_CHECK(hipGetLastError());
+        if (with_resid) HS_TRY(launch_resid_norms(c, b, x));
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
         HS_CHECK(hipStreamSynchronize(c->stream));
         if (h->done || launched >= c->prm.p_maxit) break;
@@ -909,6 +910,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st) {
         st->p_iters = h->iters;
         st->p_converged = h->converged;
         st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
+        if (with_resid) resid_from_red(h, st);
     }
     if (!std::isfinite(h->rr)) return HS_E_NAN;
     return HS_OK;
@@ -921,10 +923,11 @@ int gmg_setup(hs_ctx* c) {
     return c->prm.precision == HS_PREC_MIXED ? build_hierarchy<float>(c) : build_hierarchy<double>(c);
 }
 
-int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st) {
+int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     // the multigrid operator form needs the symmetric face-coefficient p' operator
     if (c->prm.mode != HS_MODE_CONFIG) return HS_E_ARG;
-    return c->prm.precision == HS_PREC_MIXED ? fcg<float>(c, b, x, st) : fcg<double>(c, b, x, st);
+    return c->prm.precision == HS_PREC_MIXED ? fcg<float>(c, b, x, st, with_resid)
+                                             : fcg<double>(c, b, x, st, with_resid);
 }
 
 void gmg_free(hs_ctx* c) {
