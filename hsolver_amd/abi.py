@@ -216,6 +216,7 @@ def declare(lib: C.CDLL, prefix: str) -> None:
         "create": (C.c_int, [C.POINTER(Grid), C.POINTER(Params), C.POINTER(P)]),
         "destroy": (None, [P]),
         "array_size": (I64, [P, I32]),
+        "get_params": (C.c_int, [P, C.POINTER(Params)]),
         "init": (C.c_int, [P]),
         "fluid_solve": (C.c_int, [P, PS]),
         "solid_solve": (C.c_int, [P, PS]),

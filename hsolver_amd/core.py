@@ -112,6 +112,12 @@ class Context:
     def cell_bc(self):
         self._call("cell_bc")
 
+    def run_params(self) -> abi.Params:
+        """The parameters the context runs with (by-grid choices filled in)."""
+        p = abi.Params()
+        self._call("get_params", C.byref(p))
+        return p
+
     # -- data -----------------------------------------------------------------
     def size(self, which: int) -> int:
         fn = self._fn("array_size")

@@ -526,6 +526,12 @@ void hs_destroy(hs_ctx* c) {
 
 int64_t hs_array_size(const hs_ctx* c, int32_t which) { return c ? array_len(c, which) : -1; }
 
+int hs_get_params(const hs_ctx* c, hs_params* p) {
+    if (!c || !p) return HS_E_ARG;
+    *p = c->prm;
+    return HS_OK;
+}
+
 /* FluidSolver::init fluid_solver.h:882-1237 + MobSolver::init mobs_solver.h:116-288. */
 int hs_init(hs_ctx* c) {
     if (!c) return HS_E_ARG;

@@ -179,6 +179,10 @@ const char* hs_strerror(int code);
 /* sizeof(hs_grid), sizeof(hs_params), sizeof(hs_stats): lets bindings check their layouts */
 int         hs_abi_sizes(int32_t* sizes3);
 int64_t     hs_array_size(const hs_ctx* c, int32_t which);
+/* The context's parameters as it runs them: the multigrid sweep counts and over-relaxation
+ * left to hs_create (-1 / <= 0) come back with the values it chose from the grid.  No
+ * counterpart in the reference, whose solver settings are compile-time macros. */
+int         hs_get_params(const hs_ctx* c, hs_params* p);
 
 /* ---- the reference's solver methods ---- */
 /* FluidSolver::init fluid_solver.h:882-1237 + MobSolver::init mobs_solver.h:116-288:

@@ -146,3 +146,35 @@ def test_wide_offset_sweeps_bit_equal(hs_lib, precision):
     assert res[0][:2] == res[1][:2], (res[0][:2], res[1][:2])
     for a, b in zip(res[0][2:], res[1][2:]):
         assert np.array_equal(a, b)
+
+
+def test_smoothing_chosen_by_grid(hs_lib):
+    """hs_create fills the multigrid settings left unset from the grid's cell aspect ratio:
+    one pre- and two post-sweeps on the finest level, V(3,3) below and over-relaxation
+    1.2 / 1.6 on a near-isotropic grid; V(6,6) and 1.7 everywhere on a stretched one;
+    explicit values are kept."""
+    from hsolver_amd import core
+    iso = configs.c3(32)
+    p = abi.Params.from_buffer_copy(iso.params)
+    p.precond = abi.HS_PRECOND_GMG
+    ctx = core.create(iso.grid, p, iso.q)
+    r = ctx.run_params()
+    ctx.close()
+    assert (r.mg_pre0, r.mg_post0, r.mg_pre, r.mg_post) == (1, 2, 3, 3)
+    assert (round(r.mg_omega, 4), round(r.mg_omega_coarse, 4), r.e_omega) == (1.2, 1.6, 1.0)
+    assert (r.e_pre, r.e_post) == (3, 3)
+    st = configs.c5(32, 16, 16, nfins=2)
+    assert configs.max_cell_aspect(st.grid.xv, st.grid.yv, st.grid.zv) > 8
+    p = abi.Params.from_buffer_copy(st.params)
+    p.precond = abi.HS_PRECOND_GMG
+    ctx = core.create(st.grid, p, st.q)
+    r = ctx.run_params()
+    ctx.close()
+    assert (r.mg_pre, r.mg_post, r.mg_pre0, r.mg_post0) == (6, 6, -1, -1)
+    assert (round(r.mg_omega, 4), round(r.mg_omega_coarse, 4)) == (1.7, 1.7)
+    p = abi.Params.from_buffer_copy(iso.params)
+    p.mg_pre, p.mg_post, p.mg_omega, p.mg_omega_coarse = 2, 2, 1.0, 1.0
+    ctx = core.create(iso.grid, p, iso.q)
+    r = ctx.run_params()
+    ctx.close()
+    assert (r.mg_pre, r.mg_post, r.mg_pre0, r.mg_post0, r.mg_omega, r.mg_omega_coarse) == (2, 2, -1, -1, 1.0, 1.0)
