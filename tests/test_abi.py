@@ -75,3 +75,19 @@ def test_mode_r_rejects_two_solid_regions(hs_lib):
     p = abi.params_default(abi.HS_MODE_REF)
     h = C.c_void_p()
     assert hs_lib.hs_create(C.byref(g.struct), C.byref(p), C.byref(h)) == abi.HS_E_ARG
+
+
+@pytest.mark.parametrize("shape", [(24, 12, 1), (40, 1, 1), (2, 8, 8)])
+def test_fewer_than_three_cells_refused(hs_lib, shape):
+    """hs_create refuses a grid with fewer than three cells along an axis before it touches
+    the device (HS_E_ARG), in both physics modes."""
+    import numpy as np
+    from hsolver_amd import configs
+    nx, ny, nz = shape
+    grid = abi.GridArrays(configs.uniform_axis(nx, 1.0), configs.uniform_axis(ny, 1.0), configs.uniform_axis(nz, 1.0),
+                          np.zeros(nx * ny * nz, dtype=np.uint8))
+    for mode in (abi.HS_MODE_REF, abi.HS_MODE_CONFIG):
+        p = abi.params_default(mode)
+        h = C.c_void_p()
+        assert hs_lib.hs_create(C.byref(grid.struct), C.byref(p), C.byref(h)) == abi.HS_E_ARG
+        assert not h.value
