@@ -655,15 +655,34 @@ __global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, con
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
+// Two cells per thread: 16-byte loads and stores of the FP64 vectors (hipMalloc
+// arrays, 256-byte aligned); thread 0 of block 0 takes the odd last cell.
+template <typename T> struct Pair;
+template <> struct Pair<float> { using V = float2; };
+template <> struct Pair<double> { using V = double2; };
+
 // x += alpha p, r -= alpha q; level-0 right-hand side = (T) r, iterate = 0 (unless the
 // first sweep reads it as zero); partials of <r,r>
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const double* p, const double* q,
                                                      T* b0, T* x0, int zero_x0, double* partials, const SolveCtl* ctl) {
+    using V = typename Pair<T>::V;
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double acc = 0.0;
-    GRID_STRIDE(t, n) {
+    GRID_STRIDE(t, n >> 1) {
+        const double2 xv = ((const double2*)x)[t], pv = ((const double2*)p)[t];
+        const double2 rv = ((const double2*)r)[t], qv = ((const double2*)q)[t];
+        issued_together(xv.x, xv.y, pv.x, pv.y, rv.x, rv.y, qv.x, qv.y);
+        const double r0 = rv.x - alpha * qv.x, r1 = rv.y - alpha * qv.y;
+        ((double2*)x)[t] = double2{xv.x + alpha * pv.x, xv.y + alpha * pv.y};
+        ((double2*)r)[t] = double2{r0, r1};
+        acc += r0 * r0 + r1 * r1;
+        ((V*)b0)[t] = V{(T)r0, (T)r1};
+        if (zero_x0) ((V*)x0)[t] = V{T(0), T(0)};
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long t = n - 1;
         x[t] += alpha * p[t];
         const double rv = r[t] - alpha * q[t];
         r[t] = rv;
@@ -678,12 +697,21 @@ __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* 
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_beta(long n, const T* z, const double* r, const double* q,
                                                    double* partials, const SolveCtl* ctl) {
+    using V = typename Pair<T>::V;
     if (ctl->done) return;
     double a = 0.0, b = 0.0;
-    GRID_STRIDE(t, n) {
-        const double zv = (double)z[t];
-        a += zv * r[t];
-        b += zv * q[t];
+    GRID_STRIDE(t, n >> 1) {
+        const V zv = ((const V*)z)[t];
+        const double2 rv = ((const double2*)r)[t], qv = ((const double2*)q)[t];
+        issued_together(zv.x, zv.y, rv.x, rv.y, qv.x, qv.y);
+        const double z0 = (double)zv.x, z1 = (double)zv.y;
+        a += z0 * rv.x + z1 * rv.y;
+        b += z0 * qv.x + z1 * qv.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double zv = (double)z[n - 1];
+        a += zv * r[n - 1];
+        b += zv * q[n - 1];
     }
     hsd::put_partial2<FLAT>(a, b, threadIdx.x, blockIdx.x, gridDim.x, partials);
 }
@@ -934,7 +962,7 @@ void gmg_free(hs_ctx* c) {
     dense_free(c);
     for (MGLevel* m : c->mg) {
         void* arrs[8] = {m->cw, m->cs, m->cb, m->g, m->x, m->b, m->x2, m->act};
-        for (void* a : arrs) if (a) hipFree(a);
+        for (void* a : arrs) if (a) (void)hipFree(a);
         delete m;
     }
     c->mg.clear();
