@@ -16,7 +16,10 @@
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <file.in> [iterations] [out.vtk] [--order lex|rb]\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s <file.in> [iterations] [out.vtk] [--order lex|rb] [--proc DIR]\n"
+                     "       %s <file.in> --decompose DIR   (write DIR/processor0/ and exit)\n",
+                     argv[0], argv[0]);
         return 1;
     }
     const char* in = argv[1];
@@ -24,9 +27,15 @@ int main(int argc, char** argv) {
     std::string out = "result.vtk";
     int order = HS_ORDER_LEX;
     int pos = 0;
+    const char* proc_dir = nullptr;
+    const char* decompose_dir = nullptr;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--order") && a + 1 < argc) {
             order = !std::strcmp(argv[++a], "rb") ? HS_ORDER_RB : HS_ORDER_LEX;
+        } else if (!std::strcmp(argv[a], "--proc") && a + 1 < argc) {
+            proc_dir = argv[++a];
+        } else if (!std::strcmp(argv[a], "--decompose") && a + 1 < argc) {
+            decompose_dir = argv[++a];
         } else if (pos == 0) { iters = std::atoi(argv[a]); ++pos; }
         else if (pos == 1) { out = argv[a]; ++pos; }
     }
@@ -35,6 +44,24 @@ int main(int argc, char** argv) {
     if (rc != HS_OK) {
         std::fprintf(stderr, "cannot read %s: %s\n", in, hs_strerror(rc));
         return 1;
+    }
+    if (decompose_dir) {
+        // decompose.cpp:5-31 with one part
+        rc = hs_mesh_write_proc(mesh, decompose_dir);
+        if (rc != HS_OK) std::fprintf(stderr, "cannot write %s/processor0: %s\n", decompose_dir, hs_strerror(rc));
+        else std::printf("Info: wrote %s/processor0 (cells, faces, boundary)\n", decompose_dir);
+        hs_mesh_free(mesh);
+        return rc == HS_OK ? 0 : 1;
+    }
+    if (proc_dir) {
+        // main.cpp:18-21: the rank's region cells, faces and boundaries
+        rc = hs_mesh_read_proc(mesh, proc_dir);
+        if (rc != HS_OK) {
+            std::fprintf(stderr, "cannot read %s/processor0: %s\n", proc_dir, hs_strerror(rc));
+            hs_mesh_free(mesh);
+            return 1;
+        }
+        std::printf("Info: cell types from %s/processor0\n", proc_dir);
     }
     int32_t nx, ny, nz, nmobs, nwalls, nfr, nsr;
     hs_mesh_info(mesh, &nx, &ny, &nz, &nmobs, &nwalls, &nfr, &nsr);

@@ -46,3 +46,18 @@ def test_driver_example13_matches_oracle(oracle_lib, tmp_path):
     assert rel_l2(vel[:, 0], f["u"][:, :, :nx].ravel()) < 1e-9
     assert rel_l2(vel[:, 1], f["v"][:, :ny, :].ravel()) < 1e-9
     assert rel_l2(vel[:, 2], f["w"][:nz, :, :].ravel()) < 1e-9
+
+
+def test_driver_reads_processor_files(tmp_path):
+    # main.cpp:16-21: grid from the .in, cell types from processor0/ (written here by the
+    # one-part decomposer): the same run as from the .in alone, byte for byte
+    exe = str(ROOT / "hsolver_amd" / "hsolver_main")
+    ex13 = str(ROOT / "tests" / "golden" / "example13.in")
+    r = subprocess.run([exe, ex13, "--decompose", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    a, b = tmp_path / "a.vtk", tmp_path / "b.vtk"
+    for out, extra in ((a, []), (b, ["--proc", str(tmp_path)])):
+        r = subprocess.run([exe, ex13, "2", str(out)] + extra, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr + r.stdout
+    assert "cell types from" in r.stdout
+    assert a.read_bytes() == b.read_bytes()
