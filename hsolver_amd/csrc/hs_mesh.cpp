@@ -478,7 +478,8 @@ int hs_mesh_read_proc(hs_mesh* m, const char* dir) {
             ct[c - 1] = (uint8_t)t;
         }
     if (std::find(ct.begin(), ct.end(), (uint8_t)255) != ct.end()) return HS_E_IO;
-    m->cell_type.swap(ct);
+    // in place: a grid view taken earlier (hs_mesh_grid) keeps pointing at the types
+    std::copy(ct.begin(), ct.end(), m->cell_type.begin());
     m->nfluid_regions = count_regions(*m, 0);
     m->nsolid_regions = count_regions(*m, 1);
     m->part[0] = std::move(p[0]);

@@ -46,7 +46,8 @@ int  hs_mesh_write_proc(const hs_mesh* m, const char* dir);
  * (main.cpp:18-21) and takes the cell types from its fluid and mobs cell lists.
  * HS_E_IO when a file is missing or malformed, an index is out of range, a face is not
  * a face of the cells it names, or the lists do not cover every cell exactly once (the
- * files of one rank of a multi-part decomposition do not); m is unchanged then. */
+ * files of one rank of a multi-part decomposition do not); m is unchanged then.  The
+ * types are written in place, so a grid view from hs_mesh_grid sees them. */
 int  hs_mesh_read_proc(hs_mesh* m, const char* dir);
 /* The processor view of class which (0 = fluid, 1 = mobs): as read, or as
  * hs_mesh_write_proc would write it.  A class with no cells reports ncells = 0. */

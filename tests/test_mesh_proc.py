@@ -284,3 +284,18 @@ def test_driver_decompose_then_read(tmp_path):
     assert r.returncode == 0, r.stderr
     assert files(tmp_path)["boundary"] == ("meta fluid 2 0 0\nib_0_0 364200 600\nwall 364800 15000\n\n"
                                            "meta mobs 1 0 0\nib_0_0 2700 600\n\n")
+
+
+def test_grid_view_sees_types_read_from_files(hs_lib, tmp_path):
+    from hsolver_amd import abi
+    mesh.write_proc(tmp_path, text=grid_text(6, 5, 4, mobs=[(2, 3, 2, 4, 2, 3)]), lib=hs_lib)
+    h = C.c_void_p()
+    assert hs_lib.hs_mesh_parse(grid_text(6, 5, 4).encode(), C.byref(h)) == 0
+    try:
+        g = abi.Grid()
+        hs_lib.hs_mesh_grid(h, C.byref(g))             # view taken before the read
+        assert hs_lib.hs_mesh_read_proc(h, str(tmp_path).encode()) == 0
+        ct = np.ctypeslib.as_array(g.cell_type, shape=(120,))
+        assert ct.sum() == 12
+    finally:
+        hs_lib.hs_mesh_free(h)
