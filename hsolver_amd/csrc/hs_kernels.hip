@@ -717,13 +717,6 @@ __global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* x, double* r
     put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// p = r + beta p
-__global__ __launch_bounds__(FLAT) void k_cg_pupdate(long n, const double* r, double* p, SolveCtl* ctl) {
-    if (ctl->done) return;
-    const double beta = ctl->beta;
-    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) p[t] = r[t] + beta * p[t];
-}
-
 // block partials of <a,b>
 __global__ __launch_bounds__(FLAT) void k_dot(long n, const double* a, const double* b, double* partials) {
     double acc = 0.0;
@@ -1183,15 +1176,24 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resi
     const OpArgs o = op_args(c);
     int launched = 0;
     int batch = c->last_iters > 8 ? c->last_iters - 4 : 8;
+    // p = b for the first direction (k_cg_setup); later ones, p = r + beta p, are formed
+    // inside the apply from the old direction and written to the second buffer (the same
+    // arithmetic as a separate update pass), and the buffers swap
+    double *p = c->kp, *p2 = c->kp2;
     while (true) {
         for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
-            const int slot = prof_begin(c, HS_K_SPMV);
-            k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, c->kp, c->ks, kc, gr.x, gr.y, gr.z, c->partials, c->sctl);
-            HS_TRY(prof_end(c, slot));
-            HS_TRY(launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr));
-            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, c->kp, c->ks, c->partials, c->sctl);
+            if (launched == 0) {
+                const int slot = prof_begin(c, HS_K_SPMV);
+                k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, p, c->ks, kc, gr.x, gr.y, gr.z, c->partials,
+                                                                     c->sctl);
+                HS_TRY(prof_end(c, slot));
+                HS_TRY(launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr));
+            } else {
+                HS_TRY(fcg_spmv_dot(c, c->kr, false, p, p2, c->ks));
+                std::swap(p, p2);
+            }
+            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, p, c->ks, c->partials, c->sctl);
             HS_TRY(launch_finish(c, FIN_CG_UPDATE, FLAT_BLOCKS, nullptr));
-            k_cg_pupdate<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, c->kr, c->kp, c->sctl);
         }
         HS_CHECK(hipGetLastError());
         if (with_resid) HS_TRY(launch_resid_norms(c, b, x));
