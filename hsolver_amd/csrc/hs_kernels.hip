@@ -24,6 +24,9 @@ namespace {
 using hsd::BX;
 using hsd::BY;
 using hsd::BT;
+using hsd::CBX;
+using hsd::CBY;
+using hsd::CBT;
 using hsd::FLAT;
 using hsd::FLAT_BLOCKS;
 using hsd::block_sum;
@@ -323,12 +326,12 @@ __device__ __forceinline__ int mom_update(const MomArgs& a, long i, long j, long
 // tiles go to the XCDs in contiguous runs.  Per-block partial of the squared change; the
 // finish kernel runs the control step after colour 1.
 template <int COMP>
-__global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour, int gx, int gy,
+__global__ __launch_bounds__(CBT) void k_mom_rb(MomArgs a, int ni, int nj, int nk, int kc, int colour, int gx, int gy,
                                                 int gz, double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
-    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int tid = threadIdx.x + CBX * threadIdx.y;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
-    const long ib = 1 + (long)tl.bx * (2 * BX) + 2 * threadIdx.x, j = 1 + (long)tl.by * BY + threadIdx.y;
+    const long ib = 1 + (long)tl.bx * (2 * CBX) + 2 * threadIdx.x, j = 1 + (long)tl.by * CBY + threadIdx.y;
     double acc = 0.0;
     if (ib <= ni && j <= nj) {
         const int k0 = 1 + tl.bz * kc, k1 = min(nk, k0 + kc - 1);
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(BT) void k_mom_rb(MomArgs a, int ni, int nj, int nk
             if (mom_update<COMP>(a, i, j, k, &dd) == 1) acc += dd;
         }
     }
-    put_partial<BT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
+    put_partial<CBT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
 }
 
 // One hyperplane i+j+k = s of the lexicographic sweep; dres gets the squared change.
@@ -498,12 +501,12 @@ __device__ __forceinline__ bool t_update(const TArgs& a, long i, long j, long k,
 
 // Red-black colour pass of the energy sweep over a box, two adjacent cells per thread
 // along x (every lane busy on every plane), tiles in contiguous runs per XCD.
-__global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int kc, int colour,
+__global__ __launch_bounds__(CBT) void k_t_rb(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int kc, int colour,
                                               int gx, int gy, int gz, double* partials, SweepCtl* wctl) {
     if (!wctl->active) return;
-    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int tid = threadIdx.x + CBX * threadIdx.y;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
-    const long ib = i0 + (long)tl.bx * (2 * BX) + 2 * threadIdx.x, j = j0 + (long)tl.by * BY + threadIdx.y;
+    const long ib = i0 + (long)tl.bx * (2 * CBX) + 2 * threadIdx.x, j = j0 + (long)tl.by * CBY + threadIdx.y;
     double acc = 0.0;
     if (ib < i0 + ni && j < j0 + nj) {
         const int ka = k0 + tl.bz * kc, kb = min(k0 + nk - 1, ka + kc - 1);
@@ -514,7 +517,7 @@ __global__ __launch_bounds__(BT) void k_t_rb(TArgs a, int i0, int j0, int k0, in
             if (t_update(a, i, j, k, &dd)) acc += dd;
         }
     }
-    put_partial<BT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
+    put_partial<CBT>(acc, tid, blockIdx.x, partials + (colour ? gx * gy * gz : 0));
 }
 
 __global__ __launch_bounds__(256) void k_t_plane(TArgs a, int i0, int j0, int k0, int ni, int nj, int nk, int s,
@@ -1044,14 +1047,14 @@ int momentum_sweep(hs_ctx* c, int comp) {
     int ni = d.nx, nj = d.ny, nk = d.nz;
     if (!R) { if (comp == 0) ni += 1; else if (comp == 1) nj += 1; else nk += 1; }
     if (c->prm.order == HS_ORDER_RB) {
-        const int gx = (ni + 2 * BX - 1) / (2 * BX), gy = (nj + BY - 1) / BY;
+        const int gx = (ni + 2 * CBX - 1) / (2 * CBX), gy = (nj + CBY - 1) / CBY;
         const int kc = kchunk(nk, (long)gx * gy);
         const int gz = (nk + kc - 1) / kc;
         for (int colour = 0; colour < 2; ++colour) {
             const int slot = comp == 0 ? prof_begin(c, HS_K_MOM_RB) : -1;
-            if (comp == 0) k_mom_rb<0><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
-            else if (comp == 1) k_mom_rb<1><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
-            else k_mom_rb<2><<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
+            if (comp == 0) k_mom_rb<0><<<gx * gy * gz, dim3(CBX, CBY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
+            else if (comp == 1) k_mom_rb<1><<<gx * gy * gz, dim3(CBX, CBY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
+            else k_mom_rb<2><<<gx * gy * gz, dim3(CBX, CBY), 0, c->stream>>>(a, ni, nj, nk, kc, colour, gx, gy, gz, c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
             HS_TRY(prof_end(c, slot));
         }
@@ -1078,11 +1081,11 @@ static int t_sweep_box(hs_ctx* c, bool solid, int i0, int j0, int k0, int ni, in
     const Dims& d = c->d;
     TArgs a = t_args(c, solid);
     if (c->prm.order == HS_ORDER_RB) {
-        const int gx = (ni + 2 * BX - 1) / (2 * BX), gy = (nj + BY - 1) / BY;
+        const int gx = (ni + 2 * CBX - 1) / (2 * CBX), gy = (nj + CBY - 1) / CBY;
         const int kc = kchunk(nk, (long)gx * gy);
         const int gz = (nk + kc - 1) / kc;
         for (int colour = 0; colour < 2; ++colour) {
-            k_t_rb<<<gx * gy * gz, dim3(BX, BY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, gx, gy, gz,
+            k_t_rb<<<gx * gy * gz, dim3(CBX, CBY), 0, c->stream>>>(a, i0, j0, k0, ni, nj, nk, kc, colour, gx, gy, gz,
                                                                  c->partials, c->wctl);
             HS_CHECK(hipGetLastError());
         }
