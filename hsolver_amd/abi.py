@@ -234,6 +234,7 @@ def declare(lib: C.CDLL, prefix: str) -> None:
         "put_array": (C.c_int, [P, I32, D, I64]),
         "get_fields": (C.c_int, [P, D, D, D, D, D]),
         "put_fields": (C.c_int, [P, D, D, D, D, D]),
+        "get_output": (C.c_int, [P, D, D, D, D, C.POINTER(I32)]),
         "set_source": (C.c_int, [P, D]),
         "set_conductivity": (C.c_int, [P, D]),
         "cell_bc": (C.c_int, [P]),

@@ -40,6 +40,20 @@ def hip_runtimes() -> list:
         return []
 
 
+class SolveError(RuntimeError):
+    """A forward or adjoint energy solve stopped before its tolerance (maxit reached or a
+    BiCGStab breakdown): the temperature or the gradients it would return are not solutions."""
+
+    def __init__(self, what: str, info: dict):
+        super().__init__(f"{what} did not converge: {info}")
+        self.info = info
+
+
+def _require_converged(what: str, info: dict, tol: float, maxit: int) -> None:
+    if not info["converged"] or info["breakdown"]:
+        raise SolveError(f"{what} (tol {tol:g}, maxit {maxit})", info)
+
+
 class EnergySolver:
     """A GPU context whose flow is frozen; q and k are the differentiable inputs.
 
@@ -70,6 +84,7 @@ class EnergySolver:
         info = abi.SolveInfo()
         self.ctx._call("energy_solve", C.c_double(self.tol), self.maxit, C.byref(info))
         self.last_info = info.as_dict()
+        _require_converged("energy solve", self.last_info, self.tol, self.maxit)
         T = torch.empty(self.shape, dtype=torch.float64, device=q.device)
         self.ctx._call("get_cells", abi.HS_A_TF, C.c_void_p(T.data_ptr()), abi.HS_MEM_DEVICE)
         self.version += 1
@@ -84,6 +99,7 @@ class EnergySolver:
                        C.c_void_p(dk.data_ptr()), C.c_double(self.tol), self.maxit, abi.HS_MEM_DEVICE,
                        C.byref(info))
         self.last_adjoint_info = info.as_dict()
+        _require_converged("adjoint solve", self.last_adjoint_info, self.tol, self.maxit)
         return dq, dk
 
 

@@ -17,6 +17,7 @@ SRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libhsolver_amd.so"
 EXE = PKG / "hsolver_main"
+SHIM_CHECK = PKG / "hsolver_shim_check"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HS_OFFLOAD_ARCH", "gfx950")
 
@@ -24,7 +25,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", f"-I{PKG.parent / 'include'}"]
 
-SOURCES = ["hs_kernels.hip", "hs_mg.hip", "hs_dense.hip", "hs_energy.hip", "hs_api.cpp", "hs_mesh.cpp"]
+SOURCES = ["hs_kernels.hip", "hs_mg.hip", "hs_dense.hip", "hs_energy.hip", "hs_api.cpp",
+           "hs_mesh.cpp"]
 
 
 def _compile(src: Path) -> Path:
@@ -48,12 +50,14 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # host driver (main.cpp shape) linked against the library, found next to it at run time
-    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(PKG / "host" / "hs_main.cpp"),
-           f"-L{PKG}", "-lhsolver_amd", "-Wl,-rpath,$ORIGIN", "-o", str(EXE)]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # host driver (main.cpp shape) and the shim check, linked against the library, found
+    # next to it at run time
+    for src, exe in ((PKG / "host" / "hs_main.cpp", EXE), (PKG / "host" / "hs_shim_check.cpp", SHIM_CHECK)):
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", f"-I{PKG.parent / 'include'}", str(src),
+               f"-L{PKG}", "-lhsolver_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB} and {EXE}")
     return LIB

@@ -155,23 +155,53 @@ class Context:
         return {"u": u.reshape(nz, ny, nx + 1), "v": v.reshape(nz, ny + 1, nx),
                 "w": w.reshape(nz + 1, ny, nx), "p": p.reshape(nz, ny, nx), "t": t.reshape(nz, ny, nx)}
 
+    def output(self) -> dict:
+        """The reference's output records (Solver::solve solver.h:160-213): per cell, in
+        global cell order, the minimum vertex, west/south/bottom face velocity, p, t and the
+        p' row id of fluid cells (-1 on solid cells, whose velocity and p are 0)."""
+        n = self.nx * self.ny * self.nz
+        loc = np.empty(3 * n)
+        vel = np.empty(3 * n)
+        p = np.empty(n)
+        t = np.empty(n)
+        ids = np.empty(n, dtype=np.int32)
+        self._call("get_output", abi.dptr(loc), abi.dptr(vel), abi.dptr(p), abi.dptr(t),
+                   ids.ctypes.data_as(C.POINTER(C.c_int32)))
+        return {"loc": loc.reshape(n, 3), "vel": vel.reshape(n, 3), "p": p, "t": t, "id": ids}
+
+    # cell vectors (C layout) and padded cell vectors (P layout) handed to the C side,
+    # which always reads or writes the full length: checked here before the call
+    def _cells(self, arr: np.ndarray, name: str) -> np.ndarray:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        n = self.nx * self.ny * self.nz
+        if a.size != n:
+            raise ValueError(f"{name}: expected {n} cell values, got {a.size}")
+        return a
+
+    def _padded(self, arr: np.ndarray, name: str) -> np.ndarray:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        n = (self.nx + 2) * (self.ny + 2) * (self.nz + 2)
+        if a.size != n:
+            raise ValueError(f"{name}: expected {n} padded cell values, got {a.size}")
+        return a
+
     def set_source(self, q: np.ndarray):
-        a = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        a = self._cells(q, "q")
         self._call("set_source", abi.dptr(a))
 
     def set_conductivity(self, k: np.ndarray):
-        a = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+        a = self._cells(k, "k")
         self._call("set_conductivity", abi.dptr(a))
 
     # -- p' operator ------------------------------------------------------------
     def pressure_apply(self, x: np.ndarray) -> np.ndarray:
-        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        xa = self._padded(x, "x")
         y = np.empty_like(xa)
         self._call("pressure_apply", abi.dptr(xa), abi.dptr(y))
         return y.reshape(abi.padded_shape("b", self.nx, self.ny, self.nz))
 
     def pressure_solve(self, b: np.ndarray):
-        ba = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        ba = self._padded(b, "b")
         x = np.empty_like(ba)
         st = abi.Stats()
         self._call("pressure_solve", abi.dptr(ba), abi.dptr(x), C.byref(st))
@@ -185,7 +215,7 @@ class Context:
         return info.as_dict()
 
     def adjoint_energy(self, dJdT: np.ndarray, tol: float = 1e-12, maxit: int = 500):
-        g = np.ascontiguousarray(dJdT, dtype=np.float64).reshape(-1)
+        g = self._cells(dJdT, "dJdT")
         dq = np.empty_like(g)
         dk = np.empty_like(g)
         info = abi.SolveInfo()
@@ -195,7 +225,7 @@ class Context:
         return dq.reshape(shp), dk.reshape(shp), info.as_dict()
 
     def energy_solve_rhs(self, b: np.ndarray, transpose: bool = False, tol: float = 1e-12, maxit: int = 500):
-        ba = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        ba = self._cells(b, "b")
         x = np.empty_like(ba)
         info = abi.SolveInfo()
         self._call("energy_solve_rhs", 1 if transpose else 0, ba.ctypes.data, x.ctypes.data, C.c_double(tol), maxit,
@@ -203,7 +233,7 @@ class Context:
         return x.reshape(self.nz, self.ny, self.nx), info.as_dict()
 
     def energy_apply(self, x: np.ndarray, transpose: bool = False) -> np.ndarray:
-        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        xa = self._cells(x, "x")
         y = np.empty_like(xa)
         self._call("energy_apply", 1 if transpose else 0, xa.ctypes.data, y.ctypes.data, abi.HS_MEM_HOST)
         return y.reshape(self.nz, self.ny, self.nx)
@@ -216,9 +246,7 @@ class Context:
 
     def put_cells(self, name: str, arr: np.ndarray):
         which = abi.ARRAY_NAMES.index(name)
-        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
-        if a.size != self.nx * self.ny * self.nz:
-            raise ValueError(f"{name}: expected {self.nx * self.ny * self.nz} cells, got {a.size}")
+        a = self._cells(arr, name)
         self._call("put_cells", which, a.ctypes.data, abi.HS_MEM_HOST)
 
 

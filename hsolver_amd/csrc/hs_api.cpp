@@ -225,6 +225,23 @@ int do_correct(hs_ctx* c) {
 
 }  // namespace
 
+// The solve of hs_pressure_solve on the right-hand side staged in kb: entries outside
+// fluid cells zeroed, the configured Krylov method into kx, the true residual and ||b||
+// recomputed.  The reductions use c->partials and c->sctl as they are at the call.
+int pressure_solve_staged(hs_ctx* c, hs_stats* st) {
+    HS_TRY(mask_fluid(c, c->kb));
+    hs_stats loc{};
+    const int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc)
+                                                     : cg_solve(c, c->kb, c->kx, &loc);
+    if (e != HS_OK) return e;
+    double rel, bn;
+    HS_TRY(true_relres(c, c->kb, c->kx, &rel, &bn));
+    loc.p_true_relres = rel;
+    loc.mass_imbalance = bn;
+    if (st) *st = loc;
+    return HS_OK;
+}
+
 // Records a start event if kernel is the one being timed; the pool is folded (summed
 // after a sync on its last event) when full, so any number of launches can be timed.
 int prof_begin(hs_ctx* c, int kernel) {
@@ -639,6 +656,7 @@ int hs_set_for_next(hs_ctx* c) {
 }
 
 int hs_step(hs_ctx* c, hs_stats* st) {
+    if (!c) return HS_E_ARG;
     hs_stats loc{};
     c->defer_balance = true;
     const int e = hs_fluid_solve(c, &loc);
@@ -657,6 +675,7 @@ int hs_step(hs_ctx* c, hs_stats* st) {
 }
 
 int hs_run(hs_ctx* c, int32_t n, hs_stats* st) {
+    if (!c || n < 0) return HS_E_ARG;
     for (int32_t it = 0; it < n; ++it) HS_TRY(hs_step(c, st ? st + it : nullptr));
     return HS_OK;
 }
@@ -678,14 +697,8 @@ int hs_pressure_solve(hs_ctx* c, const double* b_host, double* x_host, hs_stats*
     if (!c->have_matrix) return HS_E_STATE;
     const long n = c->d.nP;
     HS_CHECK(hipMemcpyAsync(c->kb, b_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
-    int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc) : cg_solve(c, c->kb, c->kx, &loc);
-    if (e != HS_OK) return e;
-    double rel, bn;
-    HS_TRY(true_relres(c, c->kb, c->kx, &rel, &bn));
-    loc.p_true_relres = rel;
-    loc.mass_imbalance = bn;
+    HS_TRY(pressure_solve_staged(c, &loc));
     HS_CHECK(hipMemcpyAsync(x_host, c->kx, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
     if (st) *st = loc;
@@ -739,6 +752,44 @@ int hs_get_fields(hs_ctx* c, double* u, double* v, double* w, double* p, double*
     for (long k = 0; k <= nz; ++k)
         for (long j = 0; j < ny; ++j)
             for (long i = 0; i < nx; ++i) if (w) w[i + nx * (j + ny * k)] = W[idxW(d, i + 1, j + 1, k + 1)];
+    return HS_OK;
+}
+
+/* Output packing of Solver::solve solver.h:160-213 into OutputData records
+ * (fluid_solver.h:38-45), one per cell in the order of main.cpp's resultMap (global cell
+ * index, x fastest; main.cpp:95-101).  Fluid cells: velocity on the cell's west / south /
+ * bottom faces (solver.h:172-174), p, the fluid output temperature with the domain-boundary
+ * cells at 293 (applyCellBoundaryCondition fluid_solver.h:394-404, Mode R), and the p'
+ * row number as id (fluid_solver.h:1966, :2120; 0 until a p' matrix exists, the value of
+ * the zero-filled id vector, fluid_solver.h:97).  Solid cells: the solid temperature and
+ * the OutputData defaults elsewhere (velocity and p 0, id -1; solver.h:186-197). */
+int hs_get_output(hs_ctx* c, double* loc, double* vel, double* p, double* t, int32_t* id) {
+    if (!c) return HS_E_ARG;
+    const Dims& d = c->d;
+    const long nx = d.nx, ny = d.ny, nz = d.nz;
+    std::vector<double> u((nx + 1) * ny * nz), v(nx * (ny + 1) * nz), w(nx * ny * (nz + 1)), pc(d.nC), tc(d.nC);
+    HS_TRY(hs_get_fields(c, u.data(), v.data(), w.data(), pc.data(), tc.data()));
+    const double kBoundaryT = 293.0;   // fluid_solver.h:402
+    int32_t row = 0;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                const long q = idxC(d, i, j, k);
+                const bool fluid = c->type[q] == 0;
+                if (loc) { loc[3 * q] = c->X[i]; loc[3 * q + 1] = c->Y[j]; loc[3 * q + 2] = c->Z[k]; }
+                if (vel) {
+                    vel[3 * q] = fluid ? u[i + (nx + 1) * (j + ny * k)] : 0.0;
+                    vel[3 * q + 1] = fluid ? v[i + nx * (j + (ny + 1) * k)] : 0.0;
+                    vel[3 * q + 2] = fluid ? w[i + nx * (j + ny * k)] : 0.0;
+                }
+                if (p) p[q] = fluid ? pc[q] : 0.0;
+                if (t) {
+                    const bool edge = i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1;
+                    t[q] = (fluid && edge && modeR(c)) ? kBoundaryT : tc[q];
+                }
+                if (id) id[q] = fluid ? (c->have_matrix ? row : 0) : -1;
+                if (fluid) ++row;
+            }
     return HS_OK;
 }
 
@@ -943,7 +994,7 @@ int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms) {
 
 /* active cells of each multigrid level (for byte counts); returns the level count */
 int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels) {
-    if (!c) return HS_E_ARG;
+    if (!c || max_levels < 0 || (max_levels > 0 && !dims)) return HS_E_ARG;
     int n = 0;
     for (MGLevel* m : c->mg) {
         if (n >= max_levels) break;

@@ -136,4 +136,13 @@ inline dim3 grid3(int ni, int nj, int nk, int kc) {
     return dim3((ni + BX - 1) / BX, (nj + BY - 1) / BY, (nk + kc - 1) / kc);
 }
 
+// block partials one p' Krylov solve writes at most: the flat kernels' two sums per block
+// and the two sums per block of the z-marching residual pass over the grid's columns
+inline long krylov_partials(const Dims& d) {
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kchunk(d.nz, tiles));
+    const long nb = (long)gr.x * gr.y * gr.z;
+    return 2 * nb > 2L * FLAT_BLOCKS ? 2 * nb : 2L * FLAT_BLOCKS;
+}
+
 }  // namespace hsd

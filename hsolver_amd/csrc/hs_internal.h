@@ -78,9 +78,11 @@ struct SolveCtl {
     double sdotp;
     double eps;        // tol^2
     int    iters;
-    int    done;       // 1: stop launching (converged / breakdown / maxit)
+    int    done;       // 1: stop launching (converged / breakdown / maxit / non-finite)
     int    converged;
     int    maxit;
+    int    nan;        // 1: a Krylov reduction came out NaN or Inf (the solve stopped there)
+    int    pad;
     double red[8];     // generic reduction results
 };
 
@@ -279,6 +281,8 @@ int energy_balance(hs_ctx* c, double* heat, double* resid);
 int energy_balance_async(hs_ctx* c);
 int gmg_setup(hs_ctx* c);
 int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid = false);
+// hs_pressure_solve's device part on the right-hand side staged in c->kb -> c->kx
+int pressure_solve_staged(hs_ctx* c, hs_stats* st);
 void gmg_free(hs_ctx* c);
 // dense coarsest-level solve (hs_dense.hip)
 int dense_max_cells();

@@ -755,6 +755,19 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
     double b = 0.0;
     if (kind == FIN_FCG_BETA || kind == FIN_DOT2) b = hsd::sum_partials<1024>(partials + n, n, tid);
     if (tid != 0) return;
+    // a Krylov reduction that is not finite ends the solve at once (the host reports
+    // HS_E_NAN) instead of iterating on to maxit
+    if (kind != FIN_SWEEP && kind != FIN_DOT && kind != FIN_DOT2) {
+        const bool setup = kind == FIN_CG_SETUP || kind == FIN_FCG_SETUP;
+        if (setup) ctl->nan = 0;
+        if (!isfinite(a) || !isfinite(b)) {
+            ctl->nan = 1;
+            ctl->done = 1;
+            ctl->converged = 0;
+            if (setup) { ctl->bb = a; ctl->rr = a; ctl->iters = 0; }
+            return;
+        }
+    }
     switch (kind) {
         case FIN_SWEEP: sweep_control(wctl, a); break;
         case FIN_CG_SETUP:
@@ -1212,7 +1225,7 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resi
         st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
         if (with_resid) resid_from_red(h, st);
     }
-    if (!std::isfinite(h->rr)) return HS_E_NAN;
+    if (h->nan || !std::isfinite(h->rr)) return HS_E_NAN;
     return HS_OK;
 }
 

@@ -940,7 +940,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
         st->p_relres = h->bb > 0 ? sqrt(h->rr / h->bb) : 0.0;
         if (with_resid) resid_from_red(h, st);
     }
-    if (!std::isfinite(h->rr)) return HS_E_NAN;
+    if (h->nan || !std::isfinite(h->rr)) return HS_E_NAN;
     return HS_OK;
 }
 

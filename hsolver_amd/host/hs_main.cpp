@@ -2,7 +2,11 @@
 // rank: read the `.in` mesh, build the fluid and solid solvers, run Solver::solve,
 // write result.vtk.
 //
-//   hsolver_main <file.in> [iterations=100] [out.vtk=result.vtk] [--order lex|rb]
+//   hsolver_main <file.in> [iterations=100] [out.vtk=result.vtk] [--order lex|rb] [--raw]
+//
+// result.vtk holds the reference's output records (Solver::solve solver.h:160-213 packing,
+// main.cpp:101-156 layout and number format); --raw writes the solved arrays instead, at
+// round-trip precision.
 //
 // The physics is the reference's (HS_MODE_REF, every quirk kept); decomposition is
 // collapsed to one rank and one GPU, so the processor files of the decomposer are not
@@ -14,14 +18,16 @@
 
 #include "hs_solver.hpp"
 
+static int usage(const char* prog) {
+    std::fprintf(stderr,
+                 "usage: %s <file.in> [iterations>=1] [out.vtk] [--order lex|rb] [--proc DIR] [--raw]\n"
+                 "       %s <file.in> --decompose DIR   (write DIR/processor0/ and exit)\n",
+                 prog, prog);
+    return 1;
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2) {
-        std::fprintf(stderr,
-                     "usage: %s <file.in> [iterations] [out.vtk] [--order lex|rb] [--proc DIR]\n"
-                     "       %s <file.in> --decompose DIR   (write DIR/processor0/ and exit)\n",
-                     argv[0], argv[0]);
-        return 1;
-    }
+    if (argc < 2) return usage(argv[0]);
     const char* in = argv[1];
     int iters = 100;
     std::string out = "result.vtk";
@@ -29,15 +35,33 @@ int main(int argc, char** argv) {
     int pos = 0;
     const char* proc_dir = nullptr;
     const char* decompose_dir = nullptr;
+    bool raw = false;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--order") && a + 1 < argc) {
-            order = !std::strcmp(argv[++a], "rb") ? HS_ORDER_RB : HS_ORDER_LEX;
+            const char* o = argv[++a];
+            if (!std::strcmp(o, "rb")) order = HS_ORDER_RB;
+            else if (!std::strcmp(o, "lex")) order = HS_ORDER_LEX;
+            else return usage(argv[0]);
         } else if (!std::strcmp(argv[a], "--proc") && a + 1 < argc) {
             proc_dir = argv[++a];
         } else if (!std::strcmp(argv[a], "--decompose") && a + 1 < argc) {
             decompose_dir = argv[++a];
-        } else if (pos == 0) { iters = std::atoi(argv[a]); ++pos; }
-        else if (pos == 1) { out = argv[a]; ++pos; }
+        } else if (!std::strcmp(argv[a], "--raw")) {
+            raw = true;
+        } else if (argv[a][0] == '-' && argv[a][1] == '-') {
+            return usage(argv[0]);                     // unknown flag, or a flag without its value
+        } else if (pos == 0) {
+            char* end = nullptr;
+            const long n = std::strtol(argv[a], &end, 10);
+            if (end == argv[a] || *end != '\0' || n < 1 || n > 100000000) return usage(argv[0]);
+            iters = (int)n;
+            ++pos;
+        } else if (pos == 1) {
+            out = argv[a];
+            ++pos;
+        } else {
+            return usage(argv[0]);
+        }
     }
     hs_mesh* mesh = nullptr;
     int rc = hs_mesh_read(in, &mesh);
@@ -87,7 +111,8 @@ int main(int argc, char** argv) {
         if (iters > warm)
             std::printf("Info: wall time %.3f s over iterations %d-%d, average %.4f s\n", solver.seconds_after_warmup,
                         warm, iters - 1, solver.seconds_after_warmup / (iters - warm));
-        hsolver::write_vtk(out.c_str(), ctx.get(), g);
+        if (raw) hsolver::write_vtk_raw(out.c_str(), ctx.get(), g);
+        else hsolver::write_vtk(out.c_str(), solver, nx, ny, nz);
         std::printf("VTK file written: %s\n", out.c_str());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -232,6 +232,19 @@ int hs_put_array(hs_ctx* c, int32_t which, const double* host, int64_t n);
 int hs_get_fields(hs_ctx* c, double* u, double* v, double* w, double* p, double* t);
 int hs_put_fields(hs_ctx* c, const double* u, const double* v, const double* w,
                   const double* p, const double* t);
+/* Output packing of Solver::solve solver.h:160-213: one OutputData record
+ * (fluid_solver.h:38-45) per cell, in the global cell order main.cpp:95-101 writes.
+ *   loc  3 per cell: the cell's minimum vertex (solver.h:178-180)
+ *   vel  3 per cell: fluid cells their west / south / bottom face velocity (solver.h:172-174),
+ *        solid cells 0
+ *   p    fluid cells the cell pressure, solid cells 0
+ *   t    fluid cells the fluid output temperature, with 293 on cells at the domain
+ *        boundary in HS_MODE_REF (fluid_solver.h:394-404); solid cells the solid
+ *        temperature (mobs_solver.h:290-301); HS_MODE_CONFIG: the conjugate temperature
+ *   id   fluid cells their p' row number, fluid cells counted in the same order
+ *        (fluid_solver.h:1966; 0 before the first p' assembly), solid cells -1
+ * Any pointer may be NULL. */
+int hs_get_output(hs_ctx* c, double* loc, double* vel, double* p, double* t, int32_t* id);
 /* Mode C per-cell heat source q (C layout) and conductivity k (C layout). */
 int hs_set_source(hs_ctx* c, const double* q);
 int hs_set_conductivity(hs_ctx* c, const double* k);

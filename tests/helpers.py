@@ -89,3 +89,34 @@ def export_p(ctx):
     assert lib.or_export_csr(ctx.h, C.byref(nr), C.byref(nz), rp.ctypes.data, cols.ctypes.data, vals.ctypes.data,
                              rhs.ctypes.data, cell.ctypes.data) == 0
     return sp.csr_matrix((vals, cols, rp), shape=(nr.value, nr.value)), rhs, cell
+
+
+def pack_reference(grid, st, mode_r=True, assembled=True):
+    """The reference's output records (solver.h:160-213, OutputData fluid_solver.h:38-45)
+    built in numpy from raw padded arrays `st` (Context.state()): per cell in global cell
+    order (x fastest), the minimum vertex, the west / south / bottom face velocity, p, t
+    and id.  Fluid cells: t from the fluid temperature with 293 on the domain-boundary
+    cells in Mode R (fluid_solver.h:394-404), id = p' row number counted over fluid
+    cells in the same order (initMap fluid_solver.h:1891-1905; 0 before the first
+    assembly).  Solid cells: the solid temperature (Mode R) or the conjugate one, zero
+    velocity and p, id -1."""
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    ct = np.asarray(grid.cell_type).reshape(nz, ny, nx)
+    fluid = ct == 0
+    u = st["u"][1:nz + 1, 1:ny + 1, 1:nx + 1]
+    v = st["v"][1:nz + 1, 1:ny + 1, 1:nx + 1]
+    w = st["w"][1:nz + 1, 1:ny + 1, 1:nx + 1]
+    tf = st["tf"][1:nz + 1, 1:ny + 1, 1:nx + 1]
+    ts = st["ts"][1:nz + 1, 1:ny + 1, 1:nx + 1]
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    edge = (i == 0) | (j == 0) | (k == 0) | (i == nx - 1) | (j == ny - 1) | (k == nz - 1)
+    if mode_r:
+        t = np.where(fluid, np.where(edge, 293.0, tf), ts)
+    else:
+        t = tf.copy()
+    rows = np.cumsum(fluid.ravel()) - 1
+    ids = np.where(fluid.ravel(), rows if assembled else 0, -1).astype(np.int32)
+    vel = np.stack([np.where(fluid, a, 0.0).ravel() for a in (u, v, w)], axis=1)
+    loc = np.stack([np.asarray(grid.xv)[i.ravel()], np.asarray(grid.yv)[j.ravel()],
+                    np.asarray(grid.zv)[k.ravel()]], axis=1)
+    return {"loc": loc, "vel": vel, "p": np.where(fluid, st["p"], 0.0).ravel(), "t": t.ravel(), "id": ids}

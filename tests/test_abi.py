@@ -91,3 +91,49 @@ def test_fewer_than_three_cells_refused(hs_lib, shape):
         h = C.c_void_p()
         assert hs_lib.hs_create(C.byref(grid.struct), C.byref(p), C.byref(h)) == abi.HS_E_ARG
         assert not h.value
+
+
+def test_null_context_is_an_argument_error(hs_lib):
+    """Every entry point that takes a context returns HS_E_ARG for NULL (no device is
+    touched); hs_run also refuses a negative count."""
+    P = C.c_void_p
+    calls = {
+        "hs_init": (), "hs_fluid_solve": (None,), "hs_solid_solve": (None,), "hs_exchange": (),
+        "hs_set_for_next": (), "hs_step": (None,), "hs_run": (1, None), "hs_momentum": (0, None, None),
+        "hs_pressure_correction": (None,), "hs_correct": (), "hs_fluid_energy": (None, None),
+        "hs_pressure_apply": (None, None), "hs_pressure_solve": (None, None, None), "hs_cell_bc": (),
+        "hs_get_array": (0, None, 0), "hs_put_array": (0, None, 0), "hs_get_fields": (None,) * 5,
+        "hs_put_fields": (None,) * 5, "hs_get_output": (None,) * 5, "hs_set_source": (None,),
+        "hs_set_conductivity": (None,), "hs_energy_solve": (C.c_double(1e-8), 10, None),
+        "hs_adjoint_energy": (None, None, None, C.c_double(1e-8), 10, 0, None),
+        "hs_energy_solve_rhs": (0, None, None, C.c_double(1e-8), 10, 0, None),
+        "hs_energy_apply": (0, None, None, 0), "hs_get_cells": (0, None, 0), "hs_put_cells": (0, None, 0),
+        "hs_prof_set": (0,), "hs_prof_get": (None, None), "hs_mg_levels": (None, 0), "hs_get_params": (None,),
+    }
+    for name, rest in calls.items():
+        fn = getattr(hs_lib, name)
+        assert fn(P(None), *rest) == abi.HS_E_ARG, name
+    assert hs_lib.hs_array_size.__call__  # exported
+    hs_lib.hs_array_size.restype = C.c_int64
+    assert hs_lib.hs_array_size(P(None), 0) == -1
+    hs_lib.hs_destroy(P(None))   # a no-op
+
+
+def test_wrappers_check_buffer_sizes(oracle_lib):
+    """The Python wrappers refuse a buffer whose length is not the grid's before calling
+    the C side (which always reads or writes the full length)."""
+    from hsolver_amd.core import Context
+    case = configs.c2(scale=16)
+    ctx = Context(oracle_lib, "or", case.grid, case.params)
+    nc = ctx.nx * ctx.ny * ctx.nz
+    npad = (ctx.nx + 2) * (ctx.ny + 2) * (ctx.nz + 2)
+    short_c, short_p = np.zeros(nc - 1), np.zeros(npad - 1)
+    for call in (lambda: ctx.set_source(short_c), lambda: ctx.set_conductivity(short_c),
+                 lambda: ctx.pressure_apply(short_p), lambda: ctx.pressure_solve(short_p),
+                 lambda: ctx.pressure_apply(np.zeros(nc)), lambda: ctx.adjoint_energy(short_c),
+                 lambda: ctx.energy_solve_rhs(short_c), lambda: ctx.energy_apply(short_c),
+                 lambda: ctx.put_cells("q", short_c), lambda: ctx.put("u", np.zeros(3))):
+        with pytest.raises(ValueError):
+            call()
+    ctx.set_source(np.ones(nc))   # the right size passes
+    ctx.close()

@@ -133,8 +133,7 @@ def test_autograd_function(pair):
     assert rel_l2(k.grad.cpu().numpy(), dk) < 1e-10
 
 
-@pytest.mark.skipif(__import__("os").environ.get("HS_SLOW") != "1",
-                    reason="the 128^3 oracle solves take minutes on the CPU; run with HS_SLOW=1")
+@pytest.mark.timeout(600)   # the oracle's share of this test is CPU work of 30-60 s
 def test_c4_128_full_gradient_vs_oracle(hs_lib, oracle_lib):
     """The adjoint workload at full size: converged 128^3 flow (mixed-precision SIMPLE),
     frozen; forward solve and full dJ/dq, dJ/dk of both GPU precisions against the
