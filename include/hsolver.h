@@ -219,6 +219,19 @@ int hs_pressure_apply(hs_ctx* c, const double* x_host, double* y_host);
  * ignored (x0 = 0 as in the reference, fluid_solver.h:1766). */
 int hs_pressure_solve(hs_ctx* c, const double* b_host, double* x_host, hs_stats* st);
 
+/* hs_pressure_solve for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  b and x are device pointers of the P layout on the context's GPU;
+ * the solve's control block and reduction partials live in a fixed device array of
+ * 256.5 KiB per GPU, held until the process ends, so the call allocates nothing.
+ * The result and statistics are those of hs_pressure_solve on the same input.
+ * Returns HS_E_ARG when the grid's reductions need more partials than the array holds
+ * (more than 16384 z-marching column blocks, e.g. nx*ny beyond about 4.2M cells per
+ * plane), HS_E_STATE before the p' operator exists or, for HS_PRECOND_GMG, before the
+ * first multigrid solve on the context has built its hierarchy.  Calls that overlap on
+ * one GPU (other threads, other contexts) share the array and race. */
+int hs_pressure_solve_static(hs_ctx* c, const double* b_dev, double* x_dev, hs_stats* st);
+
 /* applyCellBoundaryCondition fluid_solver.h:324-405 on its own (Mode R only):
  * boundary-cell pressure copies, ghost p_temp fill, p(0,0,0) = 0. */
 int hs_cell_bc(hs_ctx* c);
