@@ -47,6 +47,12 @@
 
 namespace {
 
+/* Planted mistakes, test infrastructure of the oracle's own pins: or_test_plant(n)
+ * makes the oracle commit mistake n (0 = none) so a test can show that the check meant
+ * to pin that function fails.  Never set outside those tests. */
+int g_plant = 0;
+inline bool planted(int n) { return g_plant == n; }
+
 struct Orc {
     int nx = 0, ny = 0, nz = 0;
     hs_params prm{};
@@ -113,6 +119,9 @@ void bc_x_ref(Orc& o) {
         }
     for (int kk = 0; kk < cz; ++kk)
         for (int j = 0; j < cy; ++j) o.u[o.IU(fx, j + 1, kk + 1)] = 0;
+    if (planted(4))
+        for (int kk = 0; kk < cz + 2; ++kk)
+            for (int j = 0; j < cy + 2; ++j) o.u[o.IU(fx + 1, j, kk)] = 0;
 }
 /* applyYBoundaryCondition fluid_solver.h:284-322 (lid on the top interior u row, Q9). */
 void bc_y_ref(Orc& o) {
@@ -122,8 +131,8 @@ void bc_y_ref(Orc& o) {
             o.u[o.IU(i, 0, kk)] = 0; o.v[o.IV(i, 0, kk)] = 0; o.w[o.IW(i, 0, kk)] = 0;
         }
     for (int kk = 0; kk < cz; ++kk)
-        for (int i = 0; i < cx; ++i) {
-            o.v[o.IV(i + 1, fy, kk + 1)] = 0;
+        for (int i = 0; i < cx + (planted(5) ? 1 : 0); ++i) {
+            if (i < cx) o.v[o.IV(i + 1, fy, kk + 1)] = 0;
             o.u[o.IU(i + 1, cy, kk + 1)] = o.prm.u_lid;
         }
 }
@@ -209,6 +218,7 @@ void bc_cfg_outflow(Orc& o, int comp) {
         for (int kk = 2; kk <= cz; ++kk)
             for (int j = 1; j <= cy; ++j) o.w[o.IW(cx + 1, j, kk)] = o.w[o.IW(cx, j, kk)];
     } else {  // temperature
+        if (planted(10)) return;
         for (int kk = 1; kk <= cz; ++kk)
             for (int j = 1; j <= cy; ++j) o.tf[o.IP(cx + 1, j, kk)] = o.tf[o.IP(cx, j, kk)];
     }
@@ -238,7 +248,7 @@ double mom_point(const Orc& o, int comp, long i, long j, long kk, bool write_onl
     const double c05 = 0.5 * o.prm.rho;
     const double c2nu = 2 * o.prm.nu;
     const double xp = o.xp[i], xw = o.xw[i], xe = o.xe[i];
-    const double yp = o.yp[j], ys = o.ys[j], yn = o.yn[j];
+    const double yp = o.yp[j], ys = (planted(11) && j == 1) ? 0.0 : o.ys[j], yn = o.yn[j];
     const double zp = o.zp[kk], zf = o.zf[kk], zb = o.zb[kk];
     const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
     const double Dw = c2nu * Ayz / (xp + xw), De = c2nu * Ayz / (xp + xe);
@@ -582,9 +592,9 @@ void correct(Orc& o) {
         for (long j = 1; j <= o.ny; ++j)
             for (long i = 1; i <= o.nx + (R ? 0 : 1); ++i) {
                 if (R ? !o.isfluid(i, j, kk) : !cfg_solved(o, 0, i, j, kk)) continue;
-                const double faceyz = o.yp[j] * o.zp[kk];
+                const double faceyz = planted(1) ? o.xp[i] * o.zp[kk] : o.yp[j] * o.zp[kk];
                 double ap = o.up[o.IU(i, j, kk)];
-                if (ap == 0) ap = 1e-10;
+                if (ap == 0 && !planted(2)) ap = 1e-10;
                 const double corr = (PP[o.IP(i - 1, j, kk)] - PP[o.IP(i, j, kk)]) * faceyz / ap;
                 o.u[o.IU(i, j, kk)] = o.u[o.IU(i, j, kk)] + corr;
             }
@@ -613,11 +623,11 @@ void correct(Orc& o) {
             for (long i = 1; i <= o.nx; ++i) {
                 if (!o.isfluid(i, j, kk)) continue;
                 const long c = o.IC(i - 1, j - 1, kk - 1);
-                o.p[c] += o.prm.relax_p * PP[o.IP(i, j, kk)];
+                o.p[c] += o.prm.relax_p * PP[o.IP(i - (planted(3) ? 1 : 0), j, kk)];
                 o.ptemp[o.IP(i, j, kk)] = o.p[c];
             }
     std::fill(o.pprime.begin(), o.pprime.end(), 0.0);
-    if (!R) bc_cfg_outflow(o, 0);
+    if (!R && !planted(12)) bc_cfg_outflow(o, 0);
 }
 
 /* ------------------------------------------------------------ energy ---- */
@@ -629,10 +639,10 @@ double temp_point_ref(const Orc& o, long i, long j, long kk) {
     const double xp = o.xp[i], xw = o.xw[i];
     const double yp = o.yp[j], ys = o.ys[j];
     const double zp = o.zp[kk], zf = o.zf[kk];
-    const double sx = (xp + xw) / 2.0, sy = (yp + ys) / 2.0, sz = (zp + zf) / 2.0;
+    const double sx = planted(7) ? (xp + o.xe[i]) / 2.0 : (xp + xw) / 2.0, sy = (yp + ys) / 2.0, sz = (zp + zf) / 2.0;
     const long c = o.IP(i, j, kk), sY = o.nx + 2, sZ = (long)(o.nx + 2) * (o.ny + 2);
     const double a = o.prm.alpha_f;
-    const double conv = -o.u[o.IU(i, j, kk)] * (T[c + 1] - T[c - 1])
+    const double conv = -o.u[o.IU(i + (planted(6) ? 1 : 0), j, kk)] * (T[c + 1] - T[c - 1])
                         - o.v[o.IV(i, j, kk)] * (T[c + sY] - T[c - sY])
                         - o.w[o.IW(i, j, kk)] * (T[c + sZ] - T[c - sZ]);
     const double diff = a * (T[c + 1] + T[c - 1]) / sx / sx
@@ -657,7 +667,7 @@ double temp_coeffs_cfg(const Orc& o, long i, long j, long kk, double a[6], doubl
         return o.k[o.IC(ii - 1, jj - 1, k2 - 1)];
     };
     auto wid = [&](const std::vector<double>& g, long n, long ii, double self) {
-        return (ii < 1 || ii > n) ? self : g[ii];
+        return (ii < 1 || ii > n) ? (planted(9) ? 0.0 : self) : g[ii];
     };
     const double dW = wid(o.xp, o.nx, i - 1, xp), dE = wid(o.xp, o.nx, i + 1, xp);
     const double dS = wid(o.yp, o.ny, j - 1, yp), dN = wid(o.yp, o.ny, j + 1, yp);
@@ -755,7 +765,7 @@ void solid_solve(Orc& o, int* sweeps, double* resid) {
             if (!o.so[c]) return;
             const double old = T[c];
             double nv = 1.0 / 6.0 * (T[c + 1] + T[c - 1] + T[c - sY] + T[c + sY] + T[c - sZ] + T[c + sZ]);
-            nv = nv * relax + (1 - relax) * old;
+            nv = planted(8) ? nv * (1 - relax) + relax * old : nv * relax + (1 - relax) * old;
             T[c] = nv;
             residual2 += (nv - old) * (nv - old);
         };
@@ -1061,6 +1071,9 @@ void bicgstab_jacobi(const Csr& A, const std::vector<double>& b, double tol, int
 }
 
 extern "C" {
+
+/* test infrastructure: plant mistake n in the oracle (0 = none); see g_plant */
+void or_test_plant(int32_t n) { g_plant = n; }
 
 int or_create(const hs_grid* g, const hs_params* p, or_ctx** out) {
     if (!g || !p || !out || g->nx < 3 || g->ny < 3 || g->nz < 3 || !g->xv || !g->yv || !g->zv || !g->cell_type)
