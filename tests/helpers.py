@@ -120,3 +120,39 @@ def pack_reference(grid, st, mode_r=True, assembled=True):
     loc = np.stack([np.asarray(grid.xv)[i.ravel()], np.asarray(grid.yv)[j.ravel()],
                     np.asarray(grid.zv)[k.ravel()]], axis=1)
     return {"loc": loc, "vel": vel, "p": np.where(fluid, st["p"], 0.0).ravel(), "t": t.ravel(), "id": ids}
+
+
+def energy_balance(case, T, u, v, w, k, q):
+    """Global energy balance of the Mode C conjugate energy equation, built from the
+    physics: heat in (sum q V) against what leaves (outflow enthalpy rho u A T_cell on the
+    x = max faces minus the inflow enthalpy rho u A T_wall on the x = 0 faces, plus
+    conduction k A (T_cell - T_wall) / width through the Dirichlet ghosts of the inlet
+    plane and the four walls; the zero-gradient outflow ghost conducts nothing).  On an
+    exactly divergence-free flow the two are equal for the exact discrete solution.  A
+    flow with a residual mass imbalance m_c per cell (net outward rho u A) adds
+    sum T_c m_c: the upwind rows carry the inflow flux on their diagonal.  T, k, q:
+    (nz, ny, nx) cell arrays; u, v, w: the padded face arrays.
+    Returns (heat in, heat out, sum T_c m_c)."""
+    p = case.params
+    nx, ny, nz = case.shape
+    g = case.grid
+    dx, dy, dz = np.diff(g.xv), np.diff(g.yv), np.diff(g.zv)
+    V = dz[:, None, None] * dy[None, :, None] * dx[None, None, :]
+    Ayz = dz[:, None] * dy[None, :]
+    Axz = dz[:, None] * dx[None, :]
+    Axy = dy[:, None] * dx[None, :]
+    Tw = p.t_wall
+    heat_in = float(np.sum(q * V))
+    out = p.rho * np.sum(u[1:nz + 1, 1:ny + 1, nx + 1] * Ayz * T[:, :, -1]) if p.outflow else 0.0
+    out -= p.rho * np.sum(u[1:nz + 1, 1:ny + 1, 1] * Ayz) * Tw
+    out += np.sum(k[:, :, 0] * Ayz * (T[:, :, 0] - Tw) / dx[0])
+    if not p.outflow:
+        out += np.sum(k[:, :, -1] * Ayz * (T[:, :, -1] - Tw) / dx[-1])
+    out += np.sum(k[:, 0, :] * Axz * (T[:, 0, :] - Tw) / dy[0]) + np.sum(k[:, -1, :] * Axz * (T[:, -1, :] - Tw) / dy[-1])
+    out += np.sum(k[0] * Axy * (T[0] - Tw) / dz[0]) + np.sum(k[-1] * Axy * (T[-1] - Tw) / dz[-1])
+    uc = u[1:nz + 1, 1:ny + 1, 1:nx + 2]
+    vc = v[1:nz + 1, 1:ny + 2, 1:nx + 1]
+    wc = w[1:nz + 2, 1:ny + 1, 1:nx + 1]
+    m = p.rho * ((uc[:, :, 1:] - uc[:, :, :-1]) * Ayz[:, :, None] + (vc[:, 1:, :] - vc[:, :-1, :]) * Axz[:, None, :]
+                 + (wc[1:] - wc[:-1]) * Axy[None, :, :])
+    return heat_in, float(out), float(np.sum(T * m))
