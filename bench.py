@@ -8,7 +8,10 @@ red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual.
                16^3 coarsest level solved with a dense FP32 inverse built on the matrix
                cores plus FP64 refinement, rebuilt at most every --dense hierarchy builds and
                sooner when the FCG iteration count rises by 2)
-  fp64_value = the same step with the V-cycle in FP64
+  fp64_value = the same step with the V-cycle in FP64 and the dense coarse inverse in FP64
+               (v_mfma_f64_16x16x4f64), rebuilt on the same rule
+The default 50 timed steps after 5 warmup steps include one rebuild of each inverse
+(hierarchy build 51), so the rate carries its amortised cost.
 Each timed step is one full SIMPLE outer iteration (x/y/z momentum, p' assembly +
 solve, velocity / pressure correction, conjugate energy sweep, set-for-next), all
 inputs resident in HBM.
@@ -45,7 +48,8 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0, fields_out=None):
+def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0, fields_out=None,
+            step_ms=None):
     from hsolver_amd import abi, core
     lib = core.library()
     lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
@@ -64,8 +68,13 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
         lib.hs_prof_set(ctx.h, prof_kernel)
     stats = []
     t0 = time.perf_counter()
+    tp = t0
     for s in range(steps):
-        stats.append(ctx.step())
+        stats.append(ctx.step())        # synchronises at its end
+        if step_ms is not None:
+            tn = time.perf_counter()
+            step_ms.append(round((tn - tp) * 1e3, 3))
+            tp = tn
     t1 = time.perf_counter()
     prof = None
     if prof_kernel:
@@ -133,16 +142,16 @@ def cpu_sample(case, n_cg_full):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="grid cells per side (the workload is 256)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 multigrid comparison run")
     ap.add_argument("--no-cg", action="store_true", help="skip the unpreconditioned FP64 CG drop-in step")
     ap.add_argument("--dense", type=int, default=50, metavar="N",
-                    help="mixed path: coarsest level (16^3) solved with the dense FP32 inverse built on the "
-                         "matrix cores, rebuilt at most every N hierarchy builds and sooner when the FCG "
-                         "iteration count rises by 2 (0: red-black sweeps there)")
+                    help="coarsest level (16^3) solved with a dense inverse built on the matrix cores (FP32 on "
+                         "the mixed path, FP64 on the FP64 one), rebuilt at most every N hierarchy builds and "
+                         "sooner when the FCG iteration count rises by 2 (0: red-black sweeps there)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
@@ -170,8 +179,9 @@ def main() -> int:
         f"(built in {time.perf_counter() - t_case:.1f} s)")
 
     f_mixed, f_fp64 = {}, {}
+    ms_mixed, ms_fp64 = [], []
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                       prof_kernel=1, dense=args.dense, fields_out=f_mixed)
+                                       prof_kernel=1, dense=args.dense, fields_out=f_mixed, step_ms=ms_mixed)
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
@@ -179,7 +189,7 @@ def main() -> int:
     st_fp64 = []
     if not args.no_fp64:
         dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                   fields_out=f_fp64)
+                                   dense=args.dense, fields_out=f_fp64, step_ms=ms_fp64)
         fp64_value = args.steps / dt64
         log(f"fp64 gmg: {fp64_value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_fp64]}")
 
@@ -223,13 +233,18 @@ def main() -> int:
                    "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
                                     "(sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
                                     if args.dense else "single-workgroup RB-GS tail"),
-                   "coarse_fp64": "single-workgroup RB-GS tail",
+                   "coarse_fp64": (f"16^3 dense FP64 inverse on MFMA (v_mfma_f64_16x16x4f64), rebuilt at most "
+                                   f"every {args.dense} builds (sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
+                                   if args.dense else "single-workgroup RB-GS tail"),
                    "sweep_order": "red-black", "physics": "configuration (inlet+lid+outflow, conjugate energy)"},
         "fp64_value": round(fp64_value, 4) if fp64_value else None,
         "mixed_over_fp64": round(value / fp64_value, 4) if fp64_value else None,
         # both paths run the same warmup + timed SIMPLE iterations from the same start, each
         # solving p' to 1e-8: relative L2 distance of the mixed path's fields from the FP64 one
         "mixed_vs_fp64_rel_l2": field_distance(f_mixed, f_fp64) if f_fp64 else None,
+        # wall time of each timed step (the step holding the inverse's rebuild stands out)
+        "step_ms_mixed": {"median": sorted(ms_mixed)[len(ms_mixed) // 2], "max": max(ms_mixed)} if ms_mixed else None,
+        "step_ms_fp64": {"median": sorted(ms_fp64)[len(ms_fp64) // 2], "max": max(ms_fp64)} if ms_fp64 else None,
         "p_iters_mixed": [s["p_iters"] for s in st_mixed],
         "p_iters_fp64": [s["p_iters"] for s in st_fp64],
         "p_true_relres_max": max(relres) if relres else None,

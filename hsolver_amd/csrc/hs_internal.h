@@ -149,14 +149,16 @@ struct EnergyState {
     int last_iters = 0;
 };
 
-// Dense inverse of the coarsest multigrid level (hs_dense.hip).
+// Dense inverse of the coarsest multigrid level (hs_dense.hip), in the scalar of the
+// hierarchy: FP32 under the mixed-precision V-cycle, FP64 under the FP64 one.
 struct DenseCoarse {
     int np = 0;                    // order, padded to a multiple of 64
-    float* M = nullptr;            // np x np, row-major: the level matrix, then its inverse
-    float* C = nullptr;            // np x 64 panel of the Gauss-Jordan step
-    float* P = nullptr;            // 64 x 64 pivot-block inverse
+    bool f64 = false;              // the matrix scalar is double (else float)
+    void* M = nullptr;             // np x np, row-major: the level matrix, then its inverse
+    void* C = nullptr;             // np x 64 panel of the Gauss-Jordan step
+    void* P = nullptr;             // 64 x 64 pivot-block inverse
     double *r = nullptr, *z = nullptr;   // right-hand side, solution (FP64)
-    float* xf = nullptr;                 // FP32 input of the inverse apply
+    void* xf = nullptr;                  // input of the inverse apply, in the matrix scalar
     long builds = 0, factorizations = 0;
     long since = 0;                // hierarchy builds since the last factorization
     int iters_ref = -1;            // FCG iterations of the first solve after it (-1: none yet)

@@ -98,8 +98,9 @@ typedef struct {
                                          = by grid: V(3,3) and V(1,2) on the finest level when no
                                          cell's aspect ratio exceeds 8, else V(6,6) everywhere */
     int32_t mg_coarse_sweeps;         /* sweeps on the coarsest level when no dense solve */
-    int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense FP32
-                                         inverse built on the matrix cores, rebuilt at most every N
+    int32_t mg_dense_coarse;          /* N > 0: coarsest level (<= 4096 cells) solved with a dense
+                                         inverse built on the matrix cores (FP32 under the mixed
+                                         V-cycle, FP64 under the FP64 one), rebuilt at most every N
                                          hierarchy builds, sooner when a p' solve takes 2 or more FCG
                                          iterations more than the first one after the last rebuild
                                          (0: red-black sweeps on the coarsest level) */
@@ -108,7 +109,9 @@ typedef struct {
     double  t_wall;                   /* Mode C: Dirichlet temperature of inlet / walls */
     int32_t fail_on_noconv;           /* 1 = return HS_E_NOCONV when the p' solve hits maxit */
     int32_t mg_dense_refine;          /* FP64 iterative-refinement steps of the dense coarse solve
-                                         (default 1) */
+                                         against the current coarse operator (default 1: it recovers
+                                         the digits the FP32 inverse loses, and the change of the
+                                         operator since either inverse was built) */
     int32_t mg_sweep_rows;            /* tile of the fused red-black sweep: 0 = by level width
                                          (256x4, 128x8 or 64x8 columns); 4, 8, 16 = 64-wide tiles
                                          of that many rows; -1 = two in-place colour passes */

@@ -1,4 +1,5 @@
-"""The dense coarsest-level solve (MFMA Gauss-Jordan inverse + FP64 refinement).
+"""The dense coarsest-level solve (MFMA Gauss-Jordan inverse, FP32 under the mixed V-cycle,
+FP64 on v_mfma_f64_16x16x4f64 under the FP64 one, each with one FP64 refinement step).
 
   * a grid of at most 4096 cells is its own coarsest level: the preconditioner is then
     the dense solve alone, so flexible CG must converge in two iterations to the
@@ -50,7 +51,9 @@ def test_dense_only_level_is_an_exact_preconditioner(hs_lib, oracle_lib, precisi
     print(f"[dense] c3 16^3 single level, precision {precision}", flush=True)
     g, b, xo = first_pressure_system(hs_lib, oracle_lib, case, gp)
     xg, sg = g.pressure_solve(b)
-    assert sg["p_converged"] == 1 and sg["p_iters"] <= 2, sg
+    # the FP64 inverse is exact to round-off: one FCG iteration; the FP32 one with its FP64
+    # refinement step: at most two
+    assert sg["p_converged"] == 1 and sg["p_iters"] <= (2 if precision == abi.HS_PREC_MIXED else 1), sg
     assert sg["p_true_relres"] < 2e-10
     assert rel_l2(xg, xo) < 1e-8
 
