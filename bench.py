@@ -23,9 +23,9 @@ launch = 28 B per cell in FP32: the three face-coupling arrays, ground, right-ha
 and the old iterate read (24 B), the new iterate written (4 B).
 
 cpu_baseline: the CPU oracle (reference algorithm: lexicographic Gauss-Seidel sweeps,
-unpreconditioned CG with HYPRE's stop rule), one core, first SIMPLE iteration of the
-same workload; the CG part is timed over a few iterations and extrapolated to the
-iteration count the same solve needs (taken from the GPU's identical CG).
+unpreconditioned CG with HYPRE's stop rule), the first SIMPLE iteration of the same
+workload timed in full, the CG on the host threads of one GPU's share (16, or fewer if
+the host has fewer), the sweeps on one thread.  A baseline, not the target.
 """
 from __future__ import annotations
 
@@ -112,31 +112,44 @@ def pmc_traffic(n):
     return None
 
 
-def cpu_sample(case, n_cg_full):
-    """Oracle, one core: first SIMPLE iteration with the CG capped at M1 and M2
-    iterations; per-CG-iteration cost from the difference, extrapolated to n_cg_full."""
+def cpu_threads() -> int:
+    """Host threads for the CPU leg: the GPU box's CPU share (16 per GPU), at most what the
+    host has."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_sample(case, threads):
+    """The oracle (the reference algorithm) on the host: the first SIMPLE iteration of the
+    workload in full, lexicographic sweeps on one thread, the p' CG (unpreconditioned, HYPRE's
+    stop rule) on `threads` z-slab threads.  Returns (seconds, CG iterations, sweep seconds)."""
     from hsolver_amd import abi
     from hsolver_amd.core import Context
     olib = C.CDLL(str(ROOT / "oracle" / "liboracle.so"))
     abi.declare(olib, "or")
-    times = {}
-    for m in (2, 6):
-        p = abi.Params.from_buffer_copy(case.params)
-        p.order = abi.HS_ORDER_LEX
-        p.precond = abi.HS_PRECOND_NONE
-        p.p_maxit = m
-        ctx = Context(olib, "or", case.grid, p)
-        ctx.set_source(case.q)
-        ctx.init()
-        t0 = time.perf_counter()
-        ctx.step()
-        times[m] = time.perf_counter() - t0
-        ctx.close()
-        log(f"cpu oracle step with CG capped at {m}: {times[m]:.2f} s")
-    t_it = max((times[6] - times[2]) / 4.0, 0.0)
-    t_rest = max(times[2] - 2 * t_it, 0.0)
-    t_full = t_rest + n_cg_full * t_it
-    return t_full, t_rest, t_it, sum(times.values())
+    olib.or_set_threads.argtypes = [C.c_int32]
+    olib.or_set_threads(threads)
+    p = abi.Params.from_buffer_copy(case.params)
+    p.order = abi.HS_ORDER_LEX
+    p.precond = abi.HS_PRECOND_NONE
+    ctx = Context(olib, "or", case.grid, p)
+    ctx.set_source(case.q)
+    ctx.init()
+    t0 = time.perf_counter()
+    for comp in range(3):
+        ctx.momentum(comp)
+    t_mom = time.perf_counter() - t0
+    st = ctx.pressure_correction()
+    ctx.correct()
+    ctx.fluid_energy()
+    ctx.solid_solve()
+    ctx.exchange()
+    ctx.set_for_next()
+    t = time.perf_counter() - t0
+    ctx.close()
+    olib.or_set_threads(1)
+    log(f"cpu oracle first step: {t:.2f} s ({st['p_iters']} CG iterations on {threads} threads, momentum "
+        f"sweeps {t_mom:.2f} s)")
+    return t, st["p_iters"], t_mom
 
 
 def main() -> int:
@@ -213,14 +226,15 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
-        n_cg = cg_step["p_iters"] if cg_step else 1000
-        t_full, t_rest, t_it, spent = cpu_sample(case, n_cg)
-        cpu = {"value": round(1.0 / t_full, 6), "unit": "SIMPLE iters/s", "cores": 1, "kind": "port",
-               "sample": (f"CPU oracle (lexicographic GS + unpreconditioned CG, HYPRE stop rule), first SIMPLE "
-                          f"iteration of the same {nx}x{ny}x{nz} workload: non-CG part {t_rest:.2f} s measured, "
-                          f"CG {t_it:.3f} s/iteration measured over 4 iterations and extrapolated to the "
-                          f"{n_cg} iterations that solve needs; {spent:.1f} s of CPU work"),
-               "s_per_iter": round(t_full, 2)}
+        nt = cpu_threads()
+        t_full, n_cg, t_mom = cpu_sample(case, nt)
+        cpu = {"value": round(1.0 / t_full, 6), "unit": "SIMPLE iters/s", "cores": nt, "kind": "port",
+               "sample": (f"CPU oracle (the reference algorithm: lexicographic Gauss-Seidel sweeps, unpreconditioned "
+                          f"CG with HYPRE's stop rule), the first SIMPLE iteration of the same {nx}x{ny}x{nz} "
+                          f"workload timed in full: {n_cg} CG iterations on {nt} threads (z-slabs, per-slab partial "
+                          f"sums), the sweeps on one thread ({t_mom:.2f} s of momentum sweeps); host "
+                          f"os.cpu_count() = {os.cpu_count()}"),
+               "s_per_iter": round(t_full, 2), "host_cpus": os.cpu_count()}
 
     relres = [s["p_true_relres"] for s in st_mixed]
     out = {

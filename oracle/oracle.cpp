@@ -521,11 +521,109 @@ double dot_fluid(const Orc& o, const std::vector<double>& a, const std::vector<d
     return s;
 }
 
+/* The CG on several host threads (or_set_threads): the same recurrence as pcg below,
+ * with the operator apply, the vector updates and the dot products split into
+ * contiguous z-slabs of the padded grid, one per thread, and each dot product summed
+ * per slab in the serial loop order, then over the slabs in slab order (deterministic
+ * for a given thread count; the rounding differs from the single-thread order).  This
+ * is the shape of the reference's z-slab MPI ranks (per-rank partial sums, then
+ * MPI_Allreduce) on one host.  Used only for the CPU baseline timing. */
+int g_threads = 1;
+
+struct Slabs {
+    int n;
+    std::vector<long> k0, k1;   // padded planes [k0, k1] of slab t (1..nz split evenly)
+};
+Slabs make_slabs(const Orc& o, int nt) {
+    Slabs s;
+    s.n = std::max(1, std::min(nt, o.nz));
+    for (int t = 0; t < s.n; ++t) {
+        s.k0.push_back(1 + (long)o.nz * t / s.n);
+        s.k1.push_back((long)o.nz * (t + 1) / s.n);
+    }
+    return s;
+}
+
+void pcg_threads(const Orc& o, const std::vector<double>& b, std::vector<double>& x, hs_stats* st) {
+    const long NP = o.NP_();
+    const Slabs sl = make_slabs(o, g_threads);
+    const long sz = (long)(o.nx + 2) * (o.ny + 2);
+    x.assign(NP, 0.0);
+    std::vector<double> r(b), p(b), s(NP, 0.0), part(sl.n, 0.0);
+    for (long c = 0; c < NP; ++c) if (!o.fl[c]) { r[c] = 0; p[c] = 0; }
+    auto pdot = [&](const std::vector<double>& a, const std::vector<double>& v) {
+#pragma omp parallel for num_threads(sl.n) schedule(static, 1)
+        for (int t = 0; t < sl.n; ++t) {
+            double acc = 0.0;
+            for (long kk = sl.k0[t]; kk <= sl.k1[t]; ++kk)
+                for (long j = 1; j <= o.ny; ++j)
+                    for (long i = 1; i <= o.nx; ++i)
+                        if (o.isfluid(i, j, kk)) { const long c = o.IP(i, j, kk); acc += a[c] * v[c]; }
+            part[t] = acc;
+        }
+        double tot = 0.0;
+        for (int t = 0; t < sl.n; ++t) tot += part[t];
+        return tot;
+    };
+    auto papply = [&](const std::vector<double>& v, std::vector<double>& y) {
+        auto X = [&](long i, long j, long kk) { return o.isfluid(i, j, kk) ? v[o.IP(i, j, kk)] : 0.0; };
+#pragma omp parallel for num_threads(sl.n) schedule(static, 1)
+        for (int t = 0; t < sl.n; ++t)
+            for (long kk = sl.k0[t]; kk <= sl.k1[t]; ++kk)
+                for (long j = 1; j <= o.ny; ++j)
+                    for (long i = 1; i <= o.nx; ++i) {
+                        const long c = o.IP(i, j, kk);
+                        if (!o.isfluid(i, j, kk)) { y[c] = 0.0; continue; }
+                        double cB, cS, cW, cE, cN, cT;
+                        if (o.modeR()) {
+                            cW = cE = o.ax[c]; cS = cN = o.ay[c]; cB = cT = o.az[c];
+                        } else {
+                            cW = o.ax[o.IU(i, j, kk)]; cE = o.ax[o.IU(i + 1, j, kk)];
+                            cS = o.ay[o.IV(i, j, kk)]; cN = o.ay[o.IV(i, j + 1, kk)];
+                            cB = o.az[o.IW(i, j, kk)]; cT = o.az[o.IW(i, j, kk + 1)];
+                        }
+                        const double d = o.modeR() ? (cE + cW + cN + cS + cT + cB) : (cW + cE + cS + cN + cB + cT);
+                        y[c] = d * v[c] - cB * X(i, j, kk - 1) - cS * X(i, j - 1, kk) - cW * X(i - 1, j, kk) -
+                               cE * X(i + 1, j, kk) - cN * X(i, j + 1, kk) - cT * X(i, j, kk + 1);
+                    }
+    };
+    // vector updates over whole slabs of the padded layout
+    auto pfor = [&](auto&& body) {
+#pragma omp parallel for num_threads(sl.n) schedule(static, 1)
+        for (int t = 0; t < sl.n; ++t)
+            for (long c = sl.k0[t] * sz; c < (sl.k1[t] + 1) * sz; ++c)
+                if (o.fl[c]) body(c);
+    };
+    const double bb = pdot(b, b);
+    if (st) { st->p_iters = 0; st->p_converged = 1; st->p_relres = 0; }
+    if (bb == 0.0) return;
+    const double eps = o.prm.p_tol * o.prm.p_tol;
+    double gamma = pdot(r, p), iprod = gamma;
+    int it = 0, conv = 0;
+    while (it + 1 <= o.prm.p_maxit) {
+        ++it;
+        papply(p, s);
+        const double sdotp = pdot(s, p);
+        if (sdotp == 0.0) break;
+        const double alpha = gamma / sdotp;
+        if (!(alpha > std::numeric_limits<double>::min())) break;
+        const double gamma_old = gamma;
+        pfor([&](long c) { x[c] += alpha * p[c]; r[c] -= alpha * s[c]; });
+        gamma = pdot(r, r);
+        iprod = gamma;
+        if (iprod / bb < eps) { conv = 1; break; }
+        const double beta = gamma / gamma_old;
+        pfor([&](long c) { p[c] = r[c] + beta * p[c]; });
+    }
+    if (st) { st->p_iters = it; st->p_converged = conv; st->p_relres = std::sqrt(iprod / bb); }
+}
+
 /* HYPRE ParCSR PCG as configured at fluid_solver.h:1871-1875 (identity
  * preconditioner, SetTwoNorm(1), SetTol, SetMaxIter, x0 = 0 per :1766/:1862),
  * restated from HYPRE's pcg.c: stop when <r,r>/<b,b> < tol^2 after an update;
  * zero <b,b> returns x = b; zero <s,p> or non-positive alpha ends the loop. */
 void pcg(const Orc& o, const std::vector<double>& b, std::vector<double>& x, hs_stats* st) {
+    if (g_threads > 1) { pcg_threads(o, b, x, st); return; }
     const long NP = o.NP_();
     x.assign(NP, 0.0);
     const double bb = dot_fluid(o, b, b);
@@ -1074,6 +1172,9 @@ extern "C" {
 
 /* test infrastructure: plant mistake n in the oracle (0 = none); see g_plant */
 void or_test_plant(int32_t n) { g_plant = n; }
+
+/* host threads of the p' CG (1, the default: the serial recurrence); see pcg_threads */
+void or_set_threads(int32_t n) { g_threads = n < 1 ? 1 : n; }
 
 int or_create(const hs_grid* g, const hs_params* p, or_ctx** out) {
     if (!g || !p || !out || g->nx < 3 || g->ny < 3 || g->nz < 3 || !g->xv || !g->yv || !g->zv || !g->cell_type)

@@ -63,6 +63,9 @@ int     or_energy_apply(or_ctx* c, int32_t transpose, const double* x, double* y
 int     or_get_cells(or_ctx* c, int32_t which, double* dst, int32_t mem);
 int     or_put_cells(or_ctx* c, int32_t which, const double* src, int32_t mem);
 
+/* host threads of the p' CG (default 1: the serial order; n > 1 splits it into z-slabs,
+ * for the CPU baseline timing) */
+void    or_set_threads(int32_t n);
 /* test infrastructure of the oracle's own pins: commit planted mistake n (0 = none) */
 void    or_test_plant(int32_t n);
 

@@ -406,3 +406,27 @@ def test_conjugate_energy_sweeps_converge_to_direct_solve(oracle_lib):
     ctx.fluid_energy()
     T = ctx.get("tf")[1:-1, 1:-1, 1:-1].ravel()
     assert np.max(np.abs(T - Ts)) < 1e-8
+
+
+def test_threaded_cg_matches_serial(oracle_lib):
+    """The oracle's multi-thread CG (the CPU baseline's) is the same recurrence split into
+    z-slabs: same stop rule, iteration count within one, solution within the tolerance."""
+    import ctypes as C
+    oracle_lib.or_set_threads.argtypes = [C.c_int32]
+    case = configs.c2(scale=4)
+    sols = {}
+    for nt in (1, 5):
+        oracle_lib.or_set_threads(nt)
+        try:
+            ctx = oracle_ctx(oracle_lib, case)
+            ctx.init()
+            run_momentum(ctx)
+            ctx.pressure_correction()
+            sols[nt] = ctx.pressure_solve(ctx.get("b"))
+            ctx.close()
+        finally:
+            oracle_lib.or_set_threads(1)
+    (x1, s1), (x5, s5) = sols[1], sols[5]
+    assert s1["p_converged"] == 1 and s5["p_converged"] == 1
+    assert abs(s1["p_iters"] - s5["p_iters"]) <= 1
+    assert np.linalg.norm(x5 - x1) / np.linalg.norm(x1) < 10 * case.params.p_tol
