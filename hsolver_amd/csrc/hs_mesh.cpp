@@ -10,6 +10,18 @@
 //   cell boxes i0,i1,j0,j1,k0,k1 (typing mesh.cpp:102-110); walls lines need 8 ints,
 //   holes lines 13 (parsed, inert as in the reference).
 //
+// Cylindrical y form (mesh.cpp:1644-1700, 2413-2450): with no `ygrid` keyword the y
+// section is `yl(1)=a, ..., nyr(1)=n, ..., yl(2)=b,`, each value read as std::stoi reads
+// the text between the key and the next comma.  The n points are (b - a) / n * i for
+// i = 0..n-1 in integer arithmetic (quirk Q25: the step truncates and the points start at
+// 0, not at a).  b - a = 360 makes y periodic (perioFlag, mesh.cpp:66-69, 145-165,
+// 322-327): n cells instead of n - 1, no y wall faces, and region adjacency wraps from
+// row 1 to row n.  The face numbering keeps the reference's formulas with NyP = n, so the
+// north face of row n is the south face of row 1 one plane up (quirk Q26, mesh.h:213-215),
+// which the processor files inherit.  The solver-side view appends one point, so row n
+// has the width of the others (the last-cell width fallback, fluid_solver.h:694-697); the
+// solver's y boundaries stay walls (fluid_solver.h has no periodic y condition).
+//
 // processor0/ files (the decomposer at one part and the solver-side reader):
 //   writer  decomposeMesh mesh.cpp:735-1581 with nParts = 1, writeProcFile :2277-2374
 //   reader  CYCASMesh::readCells / readFaces / readBoundaries mesh.cpp:2507-2642
@@ -17,6 +29,7 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <stdexcept>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -45,7 +58,9 @@ struct ProcPart {
 };
 
 struct hs_mesh {
-    std::vector<double> x, y, z;
+    std::vector<double> x, y, z;  // vertices as the solver takes them (n + 1 per direction)
+    int y_span = 0;               // yl(2) - yl(1) of the cylindrical form, 0 for ygrid
+    bool periodic_y = false;      // y_span == 360
     std::vector<std::vector<int>> mobs, walls, holes;
     std::vector<uint8_t> cell_type;
     // per global face (x faces, then y faces, then z faces, x fastest in each): 1 where a
@@ -109,15 +124,18 @@ std::string section(const std::string& content, size_t pos, const std::vector<si
 // Global face numbering of the structured grid (mesh.h:209-219, 0-based): the x faces
 // (i = 1..nx+1) first, then the y faces, then the z faces, x fastest in each.  Cells are
 // numbered from 1, x fastest (mesh.h:201-203).
+// With periodic y the reference counts NyP = Ny y-face planes (setDims mesh.cpp:60-76).
 struct FaceNum {
-    long nx, ny, nz, nxf, nyf, nzf;
+    long nx, ny, nz, nyp, nxf, nyf, nzf;
+    bool periodic;
     explicit FaceNum(const hs_mesh& m)
         : nx((long)m.x.size() - 1), ny((long)m.y.size() - 1), nz((long)m.z.size() - 1),
-          nxf((nx + 1) * ny * nz), nyf(nx * (ny + 1) * nz), nzf(nx * ny * (nz + 1)) {}
+          nyp(m.periodic_y ? ny : ny + 1), nxf((nx + 1) * ny * nz), nyf(nx * nyp * nz), nzf(nx * ny * (nz + 1)),
+          periodic(m.periodic_y) {}
     long ncells() const { return nx * ny * nz; }
     long nfaces() const { return nxf + nyf + nzf; }
     long fx(long i, long j, long k) const { return (i - 1) + (nx + 1) * ((j - 1) + ny * (k - 1)); }
-    long fy(long i, long j, long k) const { return nxf + (i - 1) + nx * ((j - 1) + (ny + 1) * (k - 1)); }
+    long fy(long i, long j, long k) const { return nxf + (i - 1) + nx * ((j - 1) + nyp * (k - 1)); }
     long fz(long i, long j, long k) const { return nxf + nyf + (i - 1) + nx * ((j - 1) + ny * (k - 1)); }
     // faces of 1-based cell c: west, east, south, north, bottom, top (getFaceIndexes mesh.h:245-253)
     void of_cell(long c, long f[6]) const {
@@ -129,13 +147,15 @@ struct FaceNum {
         f[4] = fz(i, j, k);
         f[5] = fz(i, j, k + 1);
     }
-    // the domain's outer faces, all typed Wall (initFacesType mesh.cpp:116-167)
+    // the domain's outer faces, typed Wall (initFacesType mesh.cpp:116-167); periodic y
+    // types its first and last y-face planes Internal (:145-165)
     bool outer(long f) const {
         if (f < nxf) {
             const long i = f % (nx + 1);
             return i == 0 || i == nx;
         }
         if (f < nxf + nyf) {
+            if (periodic) return false;
             const long j = ((f - nxf) / nx) % (ny + 1);
             return j == 0 || j == ny;
         }
@@ -145,7 +165,10 @@ struct FaceNum {
 };
 
 // 6-connected components of one cell class, honouring wall faces as in
-// getConnectedRegions mesh.cpp:298-454 (a wall face cuts the adjacency).
+// getConnectedRegions mesh.cpp:298-454 (a wall face cuts the adjacency).  An edge is
+// taken from a cell across that cell's own face, so with periodic y, where row n's north
+// face is another cell's face, adjacency can hold in one direction only; the walk keeps
+// that, and starts from the lowest unlabelled index as the reference's BFS does.
 int count_regions(const hs_mesh& m, uint8_t cls) {
     const FaceNum fn(m);
     const long n = fn.ncells();
@@ -153,6 +176,7 @@ int count_regions(const hs_mesh& m, uint8_t cls) {
     std::vector<long> st;
     int regions = 0;
     const long step[6] = {-1, 1, -fn.nx, fn.nx, -fn.nx * fn.ny, fn.nx * fn.ny};
+    const long wrap = fn.nx * (fn.ny - 1);  // row 1 <-> row n (mesh.cpp:322-327)
     long f[6];
     for (long c0 = 0; c0 < n; ++c0) {
         if (m.cell_type[c0] != cls || lab[c0] >= 0) continue;
@@ -162,9 +186,13 @@ int count_regions(const hs_mesh& m, uint8_t cls) {
             const long c = st.back();
             st.pop_back();
             fn.of_cell(c + 1, f);
+            const long j = (c / fn.nx) % fn.ny;
             for (int l = 0; l < 6; ++l) {
                 if (fn.outer(f[l]) || m.wall_line[f[l]]) continue;
-                const long q = c + step[l];
+                long q = c + step[l];
+                if (fn.periodic && l == 2 && j == 0) q = c + wrap;
+                if (fn.periodic && l == 3 && j == fn.ny - 1) q = c - wrap;
+                if (q < 0 || q >= n) continue;
                 if (m.cell_type[q] == cls && lab[q] < 0) { lab[q] = regions; st.push_back(q); }
             }
         }
@@ -222,6 +250,21 @@ void build_part(const hs_mesh& m, uint8_t cls, ProcPart& p) {
         p.bounds.push_back({kv.first, (int64_t)p.faces.size(), (int64_t)by_id.size()});
         for (const auto& e : by_id) p.faces.push_back(e.second);
     }
+}
+
+// One value of the cylindrical y form: the text after `key` up to the next comma, read as
+// std::stoi reads it (mesh.cpp:1682-1694); false where the reference would throw.
+bool cyl_value(const std::string& content, const char* key, int& v) {
+    const size_t at = content.find(key);
+    if (at == std::string::npos) return false;
+    const size_t from = at + std::strlen(key);
+    const size_t comma = content.find(',', at);
+    try {
+        v = std::stoi(content.substr(from, comma == std::string::npos ? std::string::npos : comma - from));
+    } catch (const std::logic_error&) {
+        return false;
+    }
+    return true;
 }
 
 std::string join(const char* dir, const char* leaf) {
@@ -363,10 +406,25 @@ int hs_mesh_parse(const char* text, hs_mesh** out) {
     const char* keys[6] = {"xgrid", "ygrid", "zgrid", "mobs", "walls", "holes"};
     std::vector<size_t> pos(6);
     for (int i = 0; i < 6; ++i) pos[i] = content.find(keys[i]);
+    // without ygrid the cylindrical form's nyr(1)= marks where the y section starts
+    const bool cylindrical = pos[1] == std::string::npos;
+    if (cylindrical) pos[1] = content.find("nyr(1)=");
     if (pos[0] == std::string::npos || pos[1] == std::string::npos || pos[2] == std::string::npos) return HS_E_IO;
+    int nyr = 0, yl1 = 0, yl2 = 0;
+    if (cylindrical && (!cyl_value(content, "nyr(1)=", nyr) || !cyl_value(content, "yl(1)=", yl1) ||
+                        !cyl_value(content, "yl(2)=", yl2) || nyr < 1))
+        return HS_E_IO;
     hs_mesh* m = new hs_mesh();
     read_pairs(section(content, pos[0], pos), m->x);
-    read_pairs(section(content, pos[1], pos), m->y);
+    if (cylindrical) {
+        const int step = (yl2 - yl1) / nyr;
+        for (int i = 0; i < nyr; ++i) m->y.push_back((double)(step * i));
+        m->y_span = yl2 - yl1;
+        m->periodic_y = m->y_span == 360;
+        if (m->periodic_y) m->y.push_back((double)(step * nyr));
+    } else {
+        read_pairs(section(content, pos[1], pos), m->y);
+    }
     read_pairs(section(content, pos[2], pos), m->z);
     read_rows(section(content, pos[3], pos), m->mobs, 0);
     read_rows(section(content, pos[4], pos), m->walls, 8);
@@ -387,7 +445,8 @@ int hs_mesh_parse(const char* text, hs_mesh** out) {
                 }
     // wall planes (initFacesType mesh.cpp:168-237); they cut region adjacency and type
     // faces for the processor files
-    const long nxf = (nx + 1) * ny * nz, nyf = nx * (ny + 1) * nz;
+    const long nyp = m->periodic_y ? ny : ny + 1;
+    const long nxf = (nx + 1) * ny * nz, nyf = nx * nyp * nz;
     m->wall_line.assign(nxf + nyf + nx * ny * (nz + 1), 0);
     uint8_t* xw = m->wall_line.data();
     uint8_t* yw = xw + nxf;
@@ -397,9 +456,9 @@ int hs_mesh_parse(const char* text, hs_mesh** out) {
         if (i0 == i1 && i0 >= 1 && i0 <= nx + 1)
             for (long k = std::max(1, k0); k <= std::min<long>(nz, k1); ++k)
                 for (long j = std::max(1, j0); j <= std::min<long>(ny, j1); ++j) xw[(i0 - 1) + (nx + 1) * ((j - 1) + ny * (k - 1))] = 1;
-        if (j0 == j1 && j0 >= 1 && j0 <= ny + 1)
+        if (j0 == j1 && j0 >= 1 && j0 <= nyp)
             for (long k = std::max(1, k0); k <= std::min<long>(nz, k1); ++k)
-                for (long i = std::max(1, i0); i <= std::min<long>(nx, i1); ++i) yw[(i - 1) + nx * ((j0 - 1) + (ny + 1) * (k - 1))] = 1;
+                for (long i = std::max(1, i0); i <= std::min<long>(nx, i1); ++i) yw[(i - 1) + nx * ((j0 - 1) + nyp * (k - 1))] = 1;
         if (k0 == k1 && k0 >= 1 && k0 <= nz + 1)
             for (long j = std::max(1, j0); j <= std::min<long>(ny, j1); ++j)
                 for (long i = std::max(1, i0); i <= std::min<long>(nx, i1); ++i) zw[(i - 1) + nx * ((j - 1) + ny * (k0 - 1))] = 1;
@@ -420,6 +479,13 @@ int hs_mesh_info(const hs_mesh* m, int32_t* nx, int32_t* ny, int32_t* nz, int32_
     if (nwalls) *nwalls = (int32_t)m->walls.size();
     if (nfluid_regions) *nfluid_regions = m->nfluid_regions;
     if (nsolid_regions) *nsolid_regions = m->nsolid_regions;
+    return HS_OK;
+}
+
+int hs_mesh_y_period(const hs_mesh* m, int32_t* span, int32_t* periodic) {
+    if (!m) return HS_E_ARG;
+    if (span) *span = m->y_span;
+    if (periodic) *periodic = m->periodic_y ? 1 : 0;
     return HS_OK;
 }
 

@@ -16,6 +16,7 @@ def _decl(lib):
     lib.hs_mesh_info.argtypes = [C.c_void_p, I, I, I, I, I, I, I]
     lib.hs_mesh_grid.argtypes = [C.c_void_p, C.POINTER(abi.Grid)]
     lib.hs_mesh_box.argtypes = [C.c_void_p, C.c_int32, I]
+    lib.hs_mesh_y_period.argtypes = [C.c_void_p, I, I]
     lib.hs_mesh_free.argtypes = [C.c_void_p]
     lib.hs_mesh_free.restype = None
     lib.hs_mesh_write_proc.argtypes = [C.c_void_p, C.c_char_p]
@@ -55,8 +56,10 @@ def _summary(lib, h) -> dict:
         box = (C.c_int32 * 6)()
         lib.hs_mesh_box(h, b, box)
         boxes.append(tuple(box))
+    span, periodic = C.c_int32(), C.c_int32()
+    lib.hs_mesh_y_period(h, C.byref(span), C.byref(periodic))
     return {"xv": xv, "yv": yv, "zv": zv, "cell_type": ct, "mobs": boxes, "nwalls": nw,
-            "fluid_regions": nf, "solid_regions": ns}
+            "fluid_regions": nf, "solid_regions": ns, "y_span": span.value, "periodic_y": bool(periodic.value)}
 
 
 def _parts(lib, h) -> dict:
@@ -80,7 +83,8 @@ def _parts(lib, h) -> dict:
 
 def read_in(path: str | None = None, text: str | None = None, lib: C.CDLL | None = None) -> dict:
     """Parse a `.in` file (or its text).  Returns vertex arrays, cell types (z,y,x),
-    the mobs boxes and the region counts."""
+    the mobs boxes, the region counts and the cylindrical y form's span and periodicity
+    (`y_span` 0 and `periodic_y` False for a ygrid file)."""
     lib, h = _open(path, text, lib)
     try:
         return _summary(lib, h)

@@ -3,7 +3,8 @@
  * input format and its single-partition cell / region typing.
  *
  * Replaces, for one rank:
- *   StructuredMesh::readMesh  mesh/mesh.cpp:1616-1893  (xgrid/ygrid/zgrid/mobs/walls/holes)
+ *   StructuredMesh::readMesh  mesh/mesh.cpp:1616-1893  (xgrid/ygrid/zgrid/mobs/walls/holes,
+ *                                                       or the cylindrical nyr(1)= y form)
  *   CYCASMesh::readMesh       mesh/mesh.cpp:2384-2488  (solver-side grid read)
  *   initCellsType             mesh/mesh.cpp:84-114     (mobs boxes -> solid cells)
  *   getConnectedRegions       mesh/mesh.cpp:298-454    (6-connected regions, walls cut)
@@ -28,6 +29,13 @@ int  hs_mesh_read(const char* path, hs_mesh** out);    /* HS_E_IO if missing/unp
 int  hs_mesh_parse(const char* text, hs_mesh** out);   /* same grammar from a string */
 int  hs_mesh_info(const hs_mesh* m, int32_t* nx, int32_t* ny, int32_t* nz, int32_t* nmobs,
                   int32_t* nwalls, int32_t* nfluid_regions, int32_t* nsolid_regions);
+/* The cylindrical y form (no ygrid; yl(1)=a, nyr(1)=n, yl(2)=b; mesh/mesh.cpp:1644-1700,
+ * perioFlag mesh/mesh.h:128): span = b - a (0 for the ygrid form), periodic = 1 when the
+ * span is 360.  A periodic mesh has n cells in y (mesh.cpp:66-69), no y wall faces and
+ * regions that wrap from row 1 to row n (mesh.cpp:145-165, 322-327), as the decomposer
+ * types it; its grid view (hs_mesh_grid) has n + 1 y vertices with row n as wide as the
+ * others, and the solver keeps walls at the y ends as the reference's solver does. */
+int  hs_mesh_y_period(const hs_mesh* m, int32_t* span, int32_t* periodic);
 /* points g at the mesh's own storage (valid until hs_mesh_free) */
 int  hs_mesh_grid(const hs_mesh* m, hs_grid* g);
 int  hs_mesh_box(const hs_mesh* m, int32_t which, int32_t* box6);
