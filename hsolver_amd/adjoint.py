@@ -150,17 +150,28 @@ def pnorm_difference(T: np.ndarray, dT: np.ndarray, solid: np.ndarray, V: np.nda
 
 def central_difference(ctx: core.Context, which: str, cell: tuple, h: float, q0: np.ndarray, k0: np.ndarray,
                        solid: np.ndarray, V: np.ndarray, tol: float = 1e-13, maxit: int = 400,
-                       p: float = 8.0) -> float:
-    """(J(x + h) - J(x - h)) / 2h for x = q or k of one cell away from the domain boundary.
+                       p: float = 8.0, shift: bool | None = None) -> float:
+    """(J(x + h) - J(x - h)) / 2h for x = q or k of one cell.
 
     The two states are not differenced directly: with A_- T_- = b_- and A_+ T_+ = b_+,
         A_+ (T_+ - T_-) = (b_+ - b_-) - (A_+ - A_-) T_-,
-    where b_+ - b_- = (q_+ - q_-) V at the cell (b does not depend on k there).  A change
-    of k at a cell away from the boundary keeps every row sum of A, so
+    where b_+ - b_- = (q_+ - q_-) V at the cell for a q change.  b does not depend on k
+    away from the boundary, nor at a boundary cell when the inlet / wall temperature
+    t_wall is 0 (its ghost term is a_face * 0); other boundary cases raise ValueError.
+    A change of k at a cell away from the boundary keeps every row sum of A, so
     (A_+ - A_-) T_- = (A_+ - A_-)(T_- - T_c), applied at k_+ and k_- to the field shifted
     by the cell's own temperature: its rounding is then relative to the local
-    temperature differences, not to T.  This is the exact nonlinear difference, with an
-    error relative to the difference itself instead of to J."""
+    temperature differences, not to T.  At a cell on the domain boundary the k change
+    also moves its boundary face's conductance and with it the row sum, so the product
+    is taken on T_- itself (shift=None picks by the cell's position; True / False
+    force it).  This is the exact nonlinear difference, with an error relative to the
+    difference itself instead of to J."""
+    nz, ny, nx = q0.shape
+    on_boundary = any(c == 0 or c == n - 1 for c, n in zip(cell, (nz, ny, nx)))
+    if shift is None:
+        shift = not on_boundary
+    if which == "k" and on_boundary and ctx.params.t_wall != 0.0:
+        raise ValueError("a k difference at a boundary cell needs t_wall = 0 (b then does not depend on k)")
     qm, qp, km, kp = q0.copy(), q0.copy(), k0.copy(), k0.copy()
     if which == "q":
         qm[cell] -= h
@@ -178,8 +189,8 @@ def central_difference(ctx: core.Context, which: str, cell: tuple, h: float, q0:
         r = np.zeros_like(Tm)
         r[cell] = (qp[cell] - qm[cell]) * V[cell]
     else:
-        shifted = Tm - Tm[cell]
-        r = ctx.energy_apply(shifted)            # A_- (T_- - T_c)
+        shifted = Tm - Tm[cell] if shift else Tm
+        r = ctx.energy_apply(shifted)            # A_- (T_- - T_c), or A_- T_-
     ctx.put_cells("q", qp)
     ctx.put_cells("k", kp)
     if which == "k":

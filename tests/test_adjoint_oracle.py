@@ -144,14 +144,19 @@ def test_adjoint_gradients_match_central_differences(frozen):
 
 def test_solve_based_central_difference_matches_matrix_difference(frozen):
     """hsolver_amd.adjoint.central_difference (operator applies + a right-hand-side solve,
-    the form the GPU check uses) against the exported-matrix difference above."""
+    the form the GPU check uses) against the exported-matrix difference above, on
+    interior, interface and boundary cells."""
     from hsolver_amd.adjoint import central_difference as solve_difference
     case, ctx = frozen
     q0, k0 = ctx.cells("q").copy(), ctx.cells("k").copy()
     solid = case.grid.cell_type.reshape(N, N, N) == 1
     V = cell_volumes(case)
     lo = int(round(N * 0.375))
-    for which, cell in (("q", (lo, lo + 1, lo)), ("k", (lo + 1, lo + 1, lo)), ("k", (lo, lo, lo - 1))):
+    # interior solid and interface cells (shifted product), then boundary cells, where the
+    # k change moves the boundary face's conductance and the product is taken unshifted:
+    # a wall cell on z = 0, one on y = 0, an outflow cell (x = max) and an inlet cell
+    for which, cell in (("q", (lo, lo + 1, lo)), ("k", (lo + 1, lo + 1, lo)), ("k", (lo, lo, lo - 1)),
+                        ("k", (0, 4, 5)), ("k", (5, 0, 3)), ("k", (3, 5, N - 1)), ("k", (4, 4, 0))):
         base = q0 if which == "q" else k0
         h = 1e-6 * base[cell]
         a = solve_difference(ctx, which, cell, h, q0, k0, solid, V, tol=1e-14, maxit=5000)
@@ -163,5 +168,27 @@ def test_solve_based_central_difference_matches_matrix_difference(frozen):
         else:
             b = central_difference(ctx, case, q0, plus, q0, minus) / (2 * h)
         assert a == pytest.approx(b, rel=1e-8), (which, cell)
+    ctx.put_cells("q", q0)
+    ctx.put_cells("k", k0)
+
+
+def test_shifted_product_is_wrong_at_a_boundary_cell(frozen):
+    """The shift is what the boundary rule switches off: forced on at a wall cell, the
+    difference misses the boundary face's change and disagrees with the matrix difference."""
+    from hsolver_amd.adjoint import central_difference as solve_difference
+    case, ctx = frozen
+    q0, k0 = ctx.cells("q").copy(), ctx.cells("k").copy()
+    solid = case.grid.cell_type.reshape(N, N, N) == 1
+    V = cell_volumes(case)
+    cell = (0, 4, 5)
+    h = 1e-6 * k0[cell]
+    plus, minus = k0.copy(), k0.copy()
+    plus[cell] += h
+    minus[cell] -= h
+    want = central_difference(ctx, case, q0, plus, q0, minus) / (2 * h)
+    good = solve_difference(ctx, "k", cell, h, q0, k0, solid, V, tol=1e-14, maxit=5000)
+    bad = solve_difference(ctx, "k", cell, h, q0, k0, solid, V, tol=1e-14, maxit=5000, shift=True)
+    assert good == pytest.approx(want, rel=1e-8)
+    assert abs(bad - want) > 1e-3 * abs(want)
     ctx.put_cells("q", q0)
     ctx.put_cells("k", k0)

@@ -3,9 +3,9 @@
   * forward solve and the matrix-free operator / transpose against the oracle's
     assembled system (FP64 and mixed-precision V-cycles);
   * dJ/dq and dJ/dk against the oracle's FP64 transpose solve (full gradients);
-  * central differences (relative step 1e-6) on 32 seeded solid cells, through the
-    GPU forward solve (J(+h) - J(-h) formed by solving for the temperature difference,
-    hsolver_amd.adjoint.central_difference);
+  * central differences (relative step 1e-6) on 32 seeded solid cells and on wall,
+    inlet, outflow and interface cells, through the GPU forward solve (J(+h) - J(-h)
+    formed by solving for the temperature difference, hsolver_amd.adjoint.central_difference);
   * the torch.autograd path (EnergySolve + solid_pnorm, J.backward()) against the
     C-ABI adjoint, including a backward after the context was re-solved.
 """
@@ -100,6 +100,34 @@ def test_central_differences_on_seeded_solid_cells(pair):
     g.put_cells("q", q0)
     g.put_cells("k", k0)
     print(f"[adjoint] worst central-difference relative error {worst}", flush=True)
+
+
+def test_central_differences_on_boundary_cells(pair):
+    """dJ/dk on cells whose rows hold a boundary face (the Dirichlet-face and outflow
+    branches of the gradient) and on fluid cells at the solid's faces, against central
+    differences of the GPU forward solve; on the boundary the difference takes the
+    unshifted product (the k change moves the boundary face's conductance)."""
+    from hsolver_amd.adjoint import central_difference
+    case, g, o = pair
+    q0, k0 = g.cells("q").copy(), g.cells("k").copy()
+    solid = case.grid.cell_type.reshape(N, N, N) == 1
+    V = cell_volumes(case)
+    g.energy_solve(tol=TOL, maxit=300)
+    _, dJdT = solid_pnorm(g.cells("tf"), solid, V)
+    _, dk, _ = g.adjoint_energy(dJdT, tol=TOL, maxit=300)
+    lo = int(round(N * 0.375))
+    m = N // 2
+    cells = [(0, m, m), (N - 1, m, m), (m, 0, m), (m, N - 1, m),    # walls z = 0 / max, y = 0 / max
+             (m, m, 0), (m, m + 1, N - 1),                           # inlet, outflow
+             (lo, lo, lo - 1), (lo - 1, lo + 1, lo + 1)]             # fluid at the solid's faces
+    # (edge cells are left out: their dJ/dk, ~1e-17 to 1e-21 here, is below what the
+    # solve tolerance resolves)
+    for cell in cells:
+        fd = central_difference(g, "k", cell, 1e-6 * k0[cell], q0, k0, solid, V, tol=TOL, maxit=300)
+        err = abs(fd - dk[cell]) / abs(dk[cell])
+        assert err < 1e-5, (cell, fd, dk[cell])
+    g.put_cells("q", q0)
+    g.put_cells("k", k0)
 
 
 def test_autograd_function(pair):
