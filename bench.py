@@ -3,7 +3,8 @@
 
 Workload (the metric's configuration): 256^3 unit box with 27 seeded solid blocks,
 conductivity ratio 1000, lid + inlet forcing at Re = 1000, configuration physics,
-red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual.
+red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual (||b - A p'|| /
+||b||), each solve starting from the previous SIMPLE iteration's p' (--cold: from zero).
   value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG; the
                16^3 coarsest level solved with a dense FP32 inverse built on the matrix
                cores plus FP64 refinement, rebuilt at most every --dense hierarchy builds and
@@ -49,7 +50,7 @@ def log(msg: str) -> None:
 
 
 def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0, fields_out=None,
-            step_ms=None):
+            step_ms=None, warm=0):
     from hsolver_amd import abi, core
     lib = core.library()
     lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
@@ -59,6 +60,7 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
     p.precond = precond
     p.order = abi.HS_ORDER_RB if order is None else order
     p.mg_dense_coarse = dense
+    p.p_warm = warm
     ctx = core.create(case.grid, p, case.q)
     ctx.init()
     for w in range(warmup):
@@ -165,6 +167,8 @@ def main() -> int:
                     help="coarsest level (16^3) solved with a dense inverse built on the matrix cores (FP32 on "
                          "the mixed path, FP64 on the FP64 one), rebuilt at most every N hierarchy builds and "
                          "sooner when the FCG iteration count rises by 2 (0: red-black sweeps there)")
+    ap.add_argument("--cold", action="store_true",
+                    help="start every multigrid p' solve from zero instead of the previous SIMPLE iteration's p'")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
@@ -194,7 +198,8 @@ def main() -> int:
     f_mixed, f_fp64 = {}, {}
     ms_mixed, ms_fp64 = [], []
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                       prof_kernel=1, dense=args.dense, fields_out=f_mixed, step_ms=ms_mixed)
+                                       prof_kernel=1, dense=args.dense, fields_out=f_mixed, step_ms=ms_mixed,
+                                       warm=0 if args.cold else 1)
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
@@ -202,7 +207,7 @@ def main() -> int:
     st_fp64 = []
     if not args.no_fp64:
         dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                   dense=args.dense, fields_out=f_fp64, step_ms=ms_fp64)
+                                   dense=args.dense, fields_out=f_fp64, step_ms=ms_fp64, warm=0 if args.cold else 1)
         fp64_value = args.steps / dt64
         log(f"fp64 gmg: {fp64_value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_fp64]}")
 
@@ -243,7 +248,8 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
-                   "p_tol": case.params.p_tol, "krylov": "flexible CG", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.2 on the finest level, 1.6 below)",
+                   "p_tol": case.params.p_tol, "krylov": "flexible CG",
+                   "p_start": "zero" if args.cold else "previous SIMPLE iteration's p'", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.2 on the finest level, 1.6 below)",
                    "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
                                     "(sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
                                     if args.dense else "single-workgroup RB-GS tail"),

@@ -48,7 +48,7 @@ class Params(C.Structure):
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
                 ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("mg_omega_coarse", C.c_float),
                 ("e_omega", C.c_float), ("mg_wide_offsets", C.c_int32), ("e_pre", C.c_int32), ("e_post", C.c_int32),
-                ("reserved", C.c_int32 * 1)]
+                ("p_warm", C.c_int32)]
 
 
 class Stats(C.Structure):

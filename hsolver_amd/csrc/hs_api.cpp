@@ -206,6 +206,7 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
     // the true residual of the solution is formed before the solver's last readback
     if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc, true);
     else e = cg_solve(c, c->b, c->kx, &loc, true);
+    c->kx_warm = e == HS_OK;
     if (e != HS_OK) return e;
     HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
     if (st) {
@@ -229,6 +230,7 @@ int do_correct(hs_ctx* c) {
 // fluid cells zeroed, the configured Krylov method into kx, the true residual and ||b||
 // recomputed.  The reductions use c->partials and c->sctl as they are at the call.
 int pressure_solve_staged(hs_ctx* c, hs_stats* st) {
+    c->kx_warm = false;   // kx is about to hold another system's solution
     HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
     const int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc)
@@ -561,6 +563,7 @@ int hs_init(hs_ctx* c) {
     }
     HS_CHECK(hipStreamSynchronize(c->stream));
     c->inited = true;
+    c->kx_warm = false;
     return HS_OK;
 }
 

@@ -171,6 +171,7 @@ struct hs_ctx {
     int device = 0;
     bool inited = false;
     bool have_matrix = false;
+    bool kx_warm = false;    // kx holds the last SIMPLE p' solution (the warm start's iterate)
     bool defer_balance = false;  // hs_step reads the energy balance at its final synchronisation
 
     // host copies
@@ -241,6 +242,7 @@ enum FinishKind : int {
     FIN_FCG_UPDATE,     // <r,r> -> stop rule
     FIN_FCG_BETA,       // <z,r>, <z,q> -> beta, gamma
     FIN_DOT2,           // two plain sums -> out[0], out[1]
+    FIN_FCG_SETUP_WARM, // <b,b>, <r,r> of a nonzero initial iterate
 };
 
 // ---- kernel launchers (defined in the .hip files) ----

@@ -753,12 +753,13 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
     const int tid = threadIdx.x;
     const double a = hsd::sum_partials<1024>(partials, n, tid);
     double b = 0.0;
-    if (kind == FIN_FCG_BETA || kind == FIN_DOT2) b = hsd::sum_partials<1024>(partials + n, n, tid);
+    if (kind == FIN_FCG_BETA || kind == FIN_DOT2 || kind == FIN_FCG_SETUP_WARM)
+        b = hsd::sum_partials<1024>(partials + n, n, tid);
     if (tid != 0) return;
     // a Krylov reduction that is not finite ends the solve at once (the host reports
     // HS_E_NAN) instead of iterating on to maxit
     if (kind != FIN_SWEEP && kind != FIN_DOT && kind != FIN_DOT2) {
-        const bool setup = kind == FIN_CG_SETUP || kind == FIN_FCG_SETUP;
+        const bool setup = kind == FIN_CG_SETUP || kind == FIN_FCG_SETUP || kind == FIN_FCG_SETUP_WARM;
         if (setup) ctl->nan = 0;
         if (!isfinite(a) || !isfinite(b)) {
             ctl->nan = 1;
@@ -794,6 +795,12 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
             ctl->bb = a; ctl->rr = a; ctl->iters = 0;
             ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
             break;
+        case FIN_FCG_SETUP_WARM: {
+            ctl->bb = a; ctl->rr = b; ctl->iters = 0;
+            const bool conv = a == 0.0 || b / a < ctl->eps;
+            ctl->converged = conv ? 1 : 0; ctl->done = conv ? 1 : 0;
+            break;
+        }
         case FIN_FCG_START: ctl->gamma = a; break;
         case FIN_FCG_UPDATE:
             ctl->iters += 1; ctl->rr = a;

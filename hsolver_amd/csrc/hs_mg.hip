@@ -641,6 +641,24 @@ __global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, dou
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
+// the setup from a nonzero iterate x (left in place) with q = A x: r = b - q; partials
+// of <b,b> and <r,r>
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_fcg_setup_warm(long n, const double* b, const double* q, double* r, T* b0,
+                                                         T* x0, int zero_x0, double* partials) {
+    double bb = 0.0, rr = 0.0;
+    GRID_STRIDE(t, n) {
+        const double bv = b[t];
+        const double rv = bv - q[t];
+        r[t] = rv;
+        b0[t] = (T)rv;
+        if (zero_x0) x0[t] = T(0);
+        bb += bv * bv;
+        rr += rv * rv;
+    }
+    hsd::put_partial2<FLAT>(bb, rr, threadIdx.x, blockIdx.x, gridDim.x, partials);
+}
+
 // p = z (the level-0 iterate); partials of <r, z>
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const T* z, double* p, double* partials,
@@ -899,8 +917,16 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     auto z = [&]() { return (T*)c->mg[0]->x; };
     const int G = FLAT_BLOCKS;
     const int zero_x0 = cycle_zero_guess(c) ? 0 : 1;
-    k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->partials);
-    HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));
+    // warm start: the previous p' solve's solution, still in x, as the initial iterate
+    const bool warm = c->prm.p_warm != 0 && c->kx_warm && x == c->kx;
+    if (warm) {
+        HS_TRY(spmv(c, x, q));
+        k_fcg_setup_warm<T><<<G, FLAT, 0, c->stream>>>(n, b, q, r, b0, z(), zero_x0, c->partials);
+        HS_TRY(launch_finish(c, FIN_FCG_SETUP_WARM, G, nullptr));
+    } else {
+        k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->partials);
+        HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));
+    }
     HS_TRY(vcycle<T>(c));
     k_fcg_start<T><<<G, FLAT, 0, c->stream>>>(n, r, z(), p, c->partials, c->sctl);
     HS_TRY(launch_finish(c, FIN_FCG_START, G, nullptr));

@@ -134,7 +134,10 @@ typedef struct {
                                          2 GiB and more (0, default: 32-bit below 2 GiB) */
     int32_t e_pre, e_post;            /* sweeps of the energy multigrid before / after its coarse
                                          correction; -1 (default) = mg_pre / mg_post */
-    int32_t reserved[1];
+    int32_t p_warm;                   /* 1 = the multigrid p' solve of a SIMPLE iteration starts from
+                                         the previous iteration's p' instead of 0 (the stop rule
+                                         stays ||r|| / ||b||); 0 (default) = from 0, as the
+                                         reference (fluid_solver.h:1766) */
 } hs_params;
 
 typedef struct {

@@ -94,11 +94,15 @@ def c3_64_oracle(oracle_lib):
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("precision", [abi.HS_PREC_FP64, abi.HS_PREC_MIXED], ids=["fp64", "mixed"])
-def test_c3_geometry_trajectory_vs_oracle(hs_lib, c3_64_oracle, precision):
+@pytest.mark.parametrize("precision,warm", [(abi.HS_PREC_FP64, 0), (abi.HS_PREC_MIXED, 0), (abi.HS_PREC_MIXED, 1)],
+                         ids=["fp64", "mixed", "mixed_warm"])
+def test_c3_geometry_trajectory_vs_oracle(hs_lib, c3_64_oracle, precision, warm):
+    """warm = 1: each p' solve starts from the previous iteration's solution (p_warm); the
+    trajectory stays within the same bound of the oracle's zero-start CG."""
     case, fo, ho = c3_64_oracle
     gp = abi.Params.from_buffer_copy(case.params)
     gp.precond, gp.precision = abi.HS_PRECOND_GMG, precision
+    gp.p_warm = warm
     gp.mg_dense_coarse = 50 if precision == abi.HS_PREC_MIXED else 0
     g = gpu_ctx(hs_lib, case, gp)
     g.init()
