@@ -4,7 +4,8 @@
 Workload (the metric's configuration): 256^3 unit box with 27 seeded solid blocks,
 conductivity ratio 1000, lid + inlet forcing at Re = 1000, configuration physics,
 red-black momentum / energy sweeps, p' solved to a 1e-8 relative residual (||b - A p'|| /
-||b||), each solve starting from the previous SIMPLE iteration's p' (--cold: from zero).
+||b||), each solve starting from the parabola through the last three SIMPLE iterations'
+p' (3 x1 - 3 x2 + x3, or zero when that is no better; --cold: from zero).
   value      = mixed-precision path (FP32 multigrid V-cycle inside FP64 flexible CG; the
                16^3 coarsest level solved with a dense FP32 inverse built on the matrix
                cores plus FP64 refinement, rebuilt at most every --dense hierarchy builds and
@@ -168,7 +169,7 @@ def main() -> int:
                          "the mixed path, FP64 on the FP64 one), rebuilt at most every N hierarchy builds and "
                          "sooner when the FCG iteration count rises by 2 (0: red-black sweeps there)")
     ap.add_argument("--cold", action="store_true",
-                    help="start every multigrid p' solve from zero instead of the previous SIMPLE iteration's p'")
+                    help="start every multigrid p' solve from zero instead of extrapolating the last SIMPLE p' solutions")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for byte counters)")
     args = ap.parse_args()
@@ -199,7 +200,7 @@ def main() -> int:
     ms_mixed, ms_fp64 = [], []
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
                                        prof_kernel=1, dense=args.dense, fields_out=f_mixed, step_ms=ms_mixed,
-                                       warm=0 if args.cold else 1)
+                                       warm=0 if args.cold else 3)
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
@@ -207,7 +208,7 @@ def main() -> int:
     st_fp64 = []
     if not args.no_fp64:
         dt64, st_fp64, _ = gpu_run(case, abi.HS_PREC_FP64, abi.HS_PRECOND_GMG, args.steps, args.warmup,
-                                   dense=args.dense, fields_out=f_fp64, step_ms=ms_fp64, warm=0 if args.cold else 1)
+                                   dense=args.dense, fields_out=f_fp64, step_ms=ms_fp64, warm=0 if args.cold else 3)
         fp64_value = args.steps / dt64
         log(f"fp64 gmg: {fp64_value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_fp64]}")
 
@@ -249,7 +250,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": case.workload, "grid": [nx, ny, nz], "solid_fraction": round(solid_frac, 4),
                    "p_tol": case.params.p_tol, "krylov": "flexible CG",
-                   "p_start": "zero" if args.cold else "previous SIMPLE iteration's p'", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.2 on the finest level, 1.6 below)",
+                   "p_start": "zero" if args.cold else "parabola through the last 3 SIMPLE p' solutions (3 x1 - 3 x2 + x3; zero if no better)", "preconditioner": "GMG, V(1,2) on the finest level and V(3,3) below, fused red-black SOR (omega 1.2 on the finest level, 1.6 below)",
                    "coarse_mixed": (f"16^3 dense FP32 inverse on MFMA, rebuilt at most every {args.dense} builds "
                                     "(sooner on an FCG iteration rise of 2), 1 FP64 refinement step"
                                     if args.dense else "single-workgroup RB-GS tail"),

@@ -206,7 +206,7 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
     // the true residual of the solution is formed before the solver's last readback
     if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc, true);
     else e = cg_solve(c, c->b, c->kx, &loc, true);
-    c->kx_warm = e == HS_OK;
+    c->kx_hist = e == HS_OK ? std::min(c->kx_hist + 1, 3) : 0;
     if (e != HS_OK) return e;
     HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
     if (st) {
@@ -230,7 +230,7 @@ int do_correct(hs_ctx* c) {
 // fluid cells zeroed, the configured Krylov method into kx, the true residual and ||b||
 // recomputed.  The reductions use c->partials and c->sctl as they are at the call.
 int pressure_solve_staged(hs_ctx* c, hs_stats* st) {
-    c->kx_warm = false;   // kx is about to hold another system's solution
+    c->kx_hist = 0;   // kx is about to hold another system's solution
     HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
     const int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc)
@@ -529,7 +529,7 @@ void hs_destroy(hs_ctx* c) {
     energy_free(c);
     double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
                       c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->kp2, c->ks, c->kx,
-                      c->kz, c->kq, c->kb, c->dres, c->partials};
+                      c->kz, c->kq, c->kb, c->dres, c->partials, c->khist[0], c->khist[1]};
     for (double* b : bufs) if (b) hipFree(b);
     if (c->cls) hipFree(c->cls);
     if (c->sctl) hipFree(c->sctl);
@@ -563,7 +563,7 @@ int hs_init(hs_ctx* c) {
     }
     HS_CHECK(hipStreamSynchronize(c->stream));
     c->inited = true;
-    c->kx_warm = false;
+    c->kx_hist = 0;
     return HS_OK;
 }
 

@@ -82,7 +82,7 @@ struct SolveCtl {
     int    converged;
     int    maxit;
     int    nan;        // 1: a Krylov reduction came out NaN or Inf (the solve stopped there)
-    int    pad;
+    int    cold;       // 1: a warm start's residual was not below ||b||: start from zero instead
     double red[8];     // generic reduction results
 };
 
@@ -171,7 +171,10 @@ struct hs_ctx {
     int device = 0;
     bool inited = false;
     bool have_matrix = false;
-    bool kx_warm = false;    // kx holds the last SIMPLE p' solution (the warm start's iterate)
+    // warm starts of the SIMPLE p' solve: the consecutive solutions held (kx = the last,
+    // khist[0] the one before, khist[1] the one before that)
+    int kx_hist = 0;
+    double* khist[2] = {nullptr, nullptr};
     bool defer_balance = false;  // hs_step reads the energy balance at its final synchronisation
 
     // host copies

@@ -796,8 +796,10 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
             ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
             break;
         case FIN_FCG_SETUP_WARM: {
-            ctl->bb = a; ctl->rr = b; ctl->iters = 0;
-            const bool conv = a == 0.0 || b / a < ctl->eps;
+            ctl->bb = a; ctl->iters = 0;
+            ctl->cold = (b < a) ? 0 : 1;     // no better than the zero start: take that
+            ctl->rr = ctl->cold ? a : b;
+            const bool conv = a == 0.0 || ctl->rr / a < ctl->eps;
             ctl->converged = conv ? 1 : 0; ctl->done = conv ? 1 : 0;
             break;
         }

@@ -659,6 +659,33 @@ __global__ __launch_bounds__(FLAT) void k_fcg_setup_warm(long n, const double* b
     hsd::put_partial2<FLAT>(bb, rr, threadIdx.x, blockIdx.x, gridDim.x, partials);
 }
 
+// the zero start after a warm one that was no better (ctl->cold): x = 0, r = b
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_fcg_cold(long n, const double* b, double* x, double* r, T* b0, T* x0,
+                                                   int zero_x0, const SolveCtl* ctl) {
+    if (!ctl->cold) return;
+    GRID_STRIDE(t, n) {
+        const double bv = b[t];
+        x[t] = 0.0;
+        r[t] = bv;
+        b0[t] = (T)bv;
+        if (zero_x0) x0[t] = T(0);
+    }
+}
+
+// out = the polynomial extrapolation of the last ord solutions x1 (newest), x2, x3:
+// ord 1: x1; 2: 2 x1 - x2; 3: 3 x1 - 3 x2 + x3 (out may be x3: each element is read
+// before it is written)
+__global__ __launch_bounds__(FLAT) void k_extrapolate(long n, int ord, const double* x1, const double* x2,
+                                                      const double* x3, double* out) {
+    GRID_STRIDE(t, n) {
+        double v = x1[t];
+        if (ord == 2) v = 2.0 * v - x2[t];
+        else if (ord == 3) v = 3.0 * v - 3.0 * x2[t] + x3[t];
+        out[t] = v;
+    }
+}
+
 // p = z (the level-0 iterate); partials of <r, z>
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const T* z, double* p, double* partials,
@@ -917,12 +944,26 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     auto z = [&]() { return (T*)c->mg[0]->x; };
     const int G = FLAT_BLOCKS;
     const int zero_x0 = cycle_zero_guess(c) ? 0 : 1;
-    // warm start: the previous p' solve's solution, still in x, as the initial iterate
-    const bool warm = c->prm.p_warm != 0 && c->kx_warm && x == c->kx;
-    if (warm) {
+    // warm start (p_warm = m): the initial iterate is the polynomial through the last
+    // min(m, 3, held) SIMPLE p' solutions (1: the last one, still in x; 2: 2 x1 - x2;
+    // 3: 3 x1 - 3 x2 + x3).  For m >= 2 it is formed in the oldest history buffer, which
+    // becomes the solve's output, and the history moves down one place.  A start whose
+    // residual is not below ||b|| falls back to zero.
+    const int ord = x == c->kx ? std::min(std::min(c->prm.p_warm, 3), c->kx_hist) : 0;
+    if (ord > 0) {
+        if (c->prm.p_warm >= 2) {
+            for (double*& h : c->khist)
+                if (!h && hipMalloc(&h, n * sizeof(double)) != hipSuccess) return HS_E_NOMEM;
+            k_extrapolate<<<G, FLAT, 0, c->stream>>>(n, ord, c->kx, c->khist[0], c->khist[1], c->khist[1]);
+            double* next = c->khist[1];
+            c->khist[1] = c->khist[0];
+            c->khist[0] = c->kx;
+            c->kx = x = next;
+        }
         HS_TRY(spmv(c, x, q));
         k_fcg_setup_warm<T><<<G, FLAT, 0, c->stream>>>(n, b, q, r, b0, z(), zero_x0, c->partials);
         HS_TRY(launch_finish(c, FIN_FCG_SETUP_WARM, G, nullptr));
+        k_fcg_cold<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->sctl);
     } else {
         k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->partials);
         HS_TRY(launch_finish(c, FIN_FCG_SETUP, G, nullptr));

@@ -134,10 +134,13 @@ typedef struct {
                                          2 GiB and more (0, default: 32-bit below 2 GiB) */
     int32_t e_pre, e_post;            /* sweeps of the energy multigrid before / after its coarse
                                          correction; -1 (default) = mg_pre / mg_post */
-    int32_t p_warm;                   /* 1 = the multigrid p' solve of a SIMPLE iteration starts from
-                                         the previous iteration's p' instead of 0 (the stop rule
-                                         stays ||r|| / ||b||); 0 (default) = from 0, as the
-                                         reference (fluid_solver.h:1766) */
+    int32_t p_warm;                   /* initial iterate of the multigrid p' solve of a SIMPLE
+                                         iteration: 0 (default) = 0, as the reference
+                                         (fluid_solver.h:1766); 1 = the previous iteration's p';
+                                         2, 3 = the line / parabola through the last 2 / 3 p'
+                                         solutions (2 x1 - x2, 3 x1 - 3 x2 + x3; fewer while fewer
+                                         are held).  The stop rule stays ||r|| / ||b|| < p_tol, and a
+                                         start no better than 0 is replaced by 0 */
 } hs_params;
 
 typedef struct {

@@ -94,11 +94,11 @@ def c3_64_oracle(oracle_lib):
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("precision,warm", [(abi.HS_PREC_FP64, 0), (abi.HS_PREC_MIXED, 0), (abi.HS_PREC_MIXED, 1)],
-                         ids=["fp64", "mixed", "mixed_warm"])
+@pytest.mark.parametrize("precision,warm", [(abi.HS_PREC_FP64, 0), (abi.HS_PREC_MIXED, 0), (abi.HS_PREC_MIXED, 3),
+                                            (abi.HS_PREC_FP64, 3)], ids=["fp64", "mixed", "mixed_warm", "fp64_warm"])
 def test_c3_geometry_trajectory_vs_oracle(hs_lib, c3_64_oracle, precision, warm):
-    """warm = 1: each p' solve starts from the previous iteration's solution (p_warm); the
-    trajectory stays within the same bound of the oracle's zero-start CG."""
+    """warm = 3: each p' solve starts from the parabola through the last three solutions
+    (p_warm); the trajectory stays within the same bound of the oracle's zero-start CG."""
     case, fo, ho = c3_64_oracle
     gp = abi.Params.from_buffer_copy(case.params)
     gp.precond, gp.precision = abi.HS_PRECOND_GMG, precision
