@@ -14,7 +14,7 @@ import numpy as np
 HS_OK, HS_E_ARG, HS_E_HIP, HS_E_NOCONV, HS_E_NAN, HS_E_IO, HS_E_STATE, HS_E_NOMEM = 0, -1, -2, -3, -4, -5, -6, -7
 HS_MODE_REF, HS_MODE_CONFIG = 0, 1
 HS_ORDER_LEX, HS_ORDER_RB = 0, 1
-HS_PRECOND_NONE, HS_PRECOND_GMG = 0, 1
+HS_PRECOND_NONE, HS_PRECOND_GMG, HS_PRECOND_AMG = 0, 1, 2
 HS_PREC_FP64, HS_PREC_MIXED = 0, 1
 HS_MEM_HOST, HS_MEM_DEVICE = 0, 1
 
@@ -48,7 +48,9 @@ class Params(C.Structure):
                 ("mg_sweep_rows", C.c_int32), ("mg_pre0", C.c_int32), ("mg_post0", C.c_int32),
                 ("mg_cycle", C.c_int32), ("mg_omega", C.c_float), ("mg_omega_coarse", C.c_float),
                 ("e_omega", C.c_float), ("mg_wide_offsets", C.c_int32), ("e_pre", C.c_int32), ("e_post", C.c_int32),
-                ("p_warm", C.c_int32)]
+                ("p_warm", C.c_int32), ("e_precond", C.c_int32), ("amg_sweeps0", C.c_int32),
+                ("amg_sweeps", C.c_int32), ("amg_theta", C.c_float), ("amg_alpha", C.c_float),
+                ("amg_omega", C.c_float), ("amg_pad", C.c_int32)]
 
 
 class Stats(C.Structure):

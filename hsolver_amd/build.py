@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", f"-I{PKG.parent / 'include'}"]
 
-SOURCES = ["hs_kernels.hip", "hs_mg.hip", "hs_dense.hip", "hs_energy.hip", "hs_static.hip", "hs_api.cpp",
+SOURCES = ["hs_kernels.hip", "hs_mg.hip", "hs_dense.hip", "hs_energy.hip", "hs_amg.hip", "hs_static.hip", "hs_api.cpp",
            "hs_mesh.cpp"]
 
 

@@ -204,7 +204,7 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
     hs_stats loc{};
     int e;
     // the true residual of the solution is formed before the solver's last readback
-    if (c->prm.precond == HS_PRECOND_GMG) e = fcg_solve(c, c->b, c->kx, &loc, true);
+    if (c->prm.precond != HS_PRECOND_NONE) e = fcg_solve(c, c->b, c->kx, &loc, true);
     else e = cg_solve(c, c->b, c->kx, &loc, true);
     c->kx_hist = e == HS_OK ? std::min(c->kx_hist + 1, 3) : 0;
     if (e != HS_OK) return e;
@@ -233,7 +233,7 @@ int pressure_solve_staged(hs_ctx* c, hs_stats* st) {
     c->kx_hist = 0;   // kx is about to hold another system's solution
     HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
-    const int e = (c->prm.precond == HS_PRECOND_GMG) ? fcg_solve(c, c->kb, c->kx, &loc)
+    const int e = (c->prm.precond != HS_PRECOND_NONE) ? fcg_solve(c, c->kb, c->kx, &loc)
                                                      : cg_solve(c, c->kb, c->kx, &loc);
     if (e != HS_OK) return e;
     double rel, bn;
@@ -406,12 +406,22 @@ static void resolve_sweeps(hs_ctx* c) {
     // 94 + 105 BiCGStab iterations on one frozen flow and left the adjoint diverging on a
     // later one
     if (!(p.e_omega > 0.0f)) p.e_omega = 1.0f;
+    // aggregation multigrid: one red-black sweep before and after on the finest level, two
+    // Jacobi sweeps on every aggregated level, theta 0.25, Gauss-Seidel on the finest level
+    // (the coarse-correction scaling stays <= 0: 1 for p', 1.6 for the energy solves)
+    if (p.amg_sweeps0 <= 0) p.amg_sweeps0 = 1;
+    if (p.amg_sweeps <= 0) p.amg_sweeps = 2;
+    if (!(p.amg_theta > 0.0f)) p.amg_theta = 0.25f;
+    if (!(p.amg_omega > 0.0f)) p.amg_omega = 1.0f;
+    if (p.e_precond == 0) p.e_precond = HS_PRECOND_GMG;
 }
 
 int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     if (!g || !p || !out || g->nx < 3 || g->ny < 3 || g->nz < 3 || !g->xv || !g->yv || !g->zv || !g->cell_type)
         return HS_E_ARG;
     if (p->mode != HS_MODE_REF && p->mode != HS_MODE_CONFIG) return HS_E_ARG;
+    if (p->precond < HS_PRECOND_NONE || p->precond > HS_PRECOND_AMG) return HS_E_ARG;
+    if (p->e_precond < 0 || p->e_precond > HS_PRECOND_AMG) return HS_E_ARG;   // 0 and GMG: geometric
     hs_ctx* c = new (std::nothrow) hs_ctx();
     if (!c) return HS_E_NOMEM;
     c->prm = *p;

@@ -253,8 +253,8 @@ __global__ __launch_bounds__(256) void k_gj_finish(S* M, int np, int kb, const S
 // FP64 across lanes.
 template <typename S>
 __global__ __launch_bounds__(256) void k_dense_gemv(const S* M, int np, const S* x, double* y, int acc,
-                                                    const SolveCtl* ctl) {
-    if (ctl->done) return;
+                                                    const int* stop) {
+    if (*stop) return;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + w;
     S s = S(0);
@@ -334,19 +334,13 @@ __global__ void k_dense_scatter(Lv<T> L, const double* z, const SolveCtl* ctl) {
     L.x[c] = (mgl::diag_of(L, c) > T(0)) ? (T)z[q] : T(0);
 }
 
-// the inverse in the level's own scalar T
+// in-place blocked Gauss-Jordan inverse of the assembled matrix D->M
 template <typename T>
-int dense_factor_t(hs_ctx* c, const MGLevel* m) {
-    DenseCoarse* D = c->dense;
+int gj_invert(hs_ctx* c, DenseCoarse* D) {
     const int np = D->np;
-    const long nn = (long)np * np;
     T* M = static_cast<T*>(D->M);
     T* P = static_cast<T*>(D->P);
     T* C = static_cast<T*>(D->C);
-    k_dense_fill<T><<<2048, 256, 0, c->stream>>>(M, nn);
-    HS_CHECK(hipGetLastError());
-    k_dense_assemble<T><<<(np + 255) / 256, 256, 0, c->stream>>>(mgl::lv<T>(m), M, np);
-    HS_CHECK(hipGetLastError());
     const int nb = np / DB;
     for (int kb = 0; kb < nb; ++kb) {
         k_gj_diag<T><<<1, 256, 0, c->stream>>>(M, np, kb, P);
@@ -358,6 +352,20 @@ int dense_factor_t(hs_ctx* c, const MGLevel* m) {
     return HS_OK;
 }
 
+// the inverse in the level's own scalar T
+template <typename T>
+int dense_factor_t(hs_ctx* c, const MGLevel* m) {
+    DenseCoarse* D = c->dense;
+    const int np = D->np;
+    const long nn = (long)np * np;
+    T* M = static_cast<T*>(D->M);
+    k_dense_fill<T><<<2048, 256, 0, c->stream>>>(M, nn);
+    HS_CHECK(hipGetLastError());
+    k_dense_assemble<T><<<(np + 255) / 256, 256, 0, c->stream>>>(mgl::lv<T>(m), M, np);
+    HS_CHECK(hipGetLastError());
+    return gj_invert<T>(c, D);
+}
+
 template <typename T>
 int dense_solve_t(hs_ctx* c, const MGLevel* m) {
     DenseCoarse* D = c->dense;
@@ -367,17 +375,139 @@ int dense_solve_t(hs_ctx* c, const MGLevel* m) {
     T* xf = static_cast<T*>(D->xf);
     const int gb = (np + 255) / 256;
     k_dense_gather<T><<<gb, 256, 0, c->stream>>>(L, D->r, xf, np, c->sctl);
-    k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 0, c->sctl);
+    k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 0, &c->sctl->done);
     for (int s = 0; s < c->prm.mg_dense_refine; ++s) {
         k_dense_resid<T><<<gb, 256, 0, c->stream>>>(L, D->z, D->r, xf, np, c->sctl);
-        k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 1, c->sctl);
+        k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 1, &c->sctl->done);
     }
     k_dense_scatter<T><<<gb, 256, 0, c->stream>>>(L, D->z, c->sctl);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 
+
+// ---- CSR levels (aggregation multigrid) -------------------------------------------------
+// row r: diagonal dd[r], off-diagonal entries vd[rp[r] .. rp[r+1]) in columns ci; rows n..np
+// of the padding are identity rows
+template <typename T>
+__global__ void k_dense_assemble_csr(int n, const int* rp, const int* ci, const double* vd, const double* dd, T* M,
+                                     int np) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    T* row = M + (long)r * np;
+    if (r >= n) { row[r] = T(1); return; }
+    row[r] = (T)dd[r];
+    for (int t = rp[r]; t < rp[r + 1]; ++t) row[ci[t]] = (T)vd[t];
+}
+
+// r = b (FP64), xf = b in the inverse's scalar; padding rows 0
+template <typename T>
+__global__ void k_dense_gather_csr(int n, const T* b, double* r, T* xf, int np, const int* skip) {
+    if (*skip) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const double v = q < n ? (double)b[q] : 0.0;
+    r[q] = v;
+    xf[q] = (T)v;
+}
+
+// e = r - A z in FP64 with the level's FP64 entries
+template <typename T>
+__global__ void k_dense_resid_csr(int n, const int* rp, const int* ci, const double* vd, const double* dd,
+                                  const double* z, const double* r, T* e, int np, const int* skip) {
+    if (*skip) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    double v = 0.0;
+    if (q < n) {
+        double az = dd[q] * z[q];
+        for (int t = rp[q]; t < rp[q + 1]; ++t) az += vd[t] * z[ci[t]];
+        v = r[q] - az;
+    }
+    e[q] = (T)v;
+}
+
+template <typename T>
+__global__ void k_dense_scatter_csr(int n, const double* z, T* x, const int* skip) {
+    if (*skip) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = (T)z[q];
+}
+
+int dense_alloc(DenseCoarse** Dp, int np, bool f32) {
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    DenseCoarse* D = *Dp;
+    if (D && (D->np != np || D->f64 == f32)) dense_release(Dp), D = nullptr;
+    if (D) return HS_OK;
+    D = new DenseCoarse();
+    *Dp = D;
+    D->np = np;
+    D->f64 = !f32;
+    if (hipMalloc(&D->M, (size_t)np * np * es) != hipSuccess) return HS_E_NOMEM;
+    if (hipMalloc(&D->C, (size_t)np * DB * es) != hipSuccess) return HS_E_NOMEM;
+    if (hipMalloc(&D->P, (size_t)DB * DB * es) != hipSuccess) return HS_E_NOMEM;
+    double** v[2] = {&D->r, &D->z};
+    for (double** a : v)
+        if (hipMalloc(a, (size_t)np * sizeof(double)) != hipSuccess) return HS_E_NOMEM;
+    if (hipMalloc(&D->xf, (size_t)np * es) != hipSuccess) return HS_E_NOMEM;
+    return HS_OK;
+}
+
+template <typename T>
+int dense_factor_csr_t(hs_ctx* c, DenseCoarse* D, int n, const int* rp, const int* ci, const double* vd,
+                       const double* dd) {
+    const int np = D->np;
+    T* M = static_cast<T*>(D->M);
+    k_dense_fill<T><<<2048, 256, 0, c->stream>>>(M, (long)np * np);
+    HS_CHECK(hipGetLastError());
+    k_dense_assemble_csr<T><<<(np + 255) / 256, 256, 0, c->stream>>>(n, rp, ci, vd, dd, M, np);
+    HS_CHECK(hipGetLastError());
+    return gj_invert<T>(c, D);
+}
+
+template <typename T>
+int dense_solve_csr_t(hs_ctx* c, DenseCoarse* D, int n, const int* rp, const int* ci, const double* vd,
+                      const double* dd, const T* b, T* x, int refine, const int* skip) {
+    const int np = D->np;
+    const T* M = static_cast<const T*>(D->M);
+    T* xf = static_cast<T*>(D->xf);
+    const int gb = (np + 255) / 256;
+    k_dense_gather_csr<T><<<gb, 256, 0, c->stream>>>(n, b, D->r, xf, np, skip);
+    k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 0, skip);
+    for (int s = 0; s < refine; ++s) {
+        k_dense_resid_csr<T><<<gb, 256, 0, c->stream>>>(n, rp, ci, vd, dd, D->z, D->r, xf, np, skip);
+        k_dense_gemv<T><<<np / 4, 256, 0, c->stream>>>(M, np, xf, D->z, 1, skip);
+    }
+    k_dense_scatter_csr<T><<<gb, 256, 0, c->stream>>>(n, D->z, x, skip);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+
 }  // namespace
+
+int dense_factor_csr(hs_ctx* c, DenseCoarse** D, int n, const int* rp, const int* ci, const double* vd,
+                     const double* dd, bool f32) {
+    if (n < 1 || n > DENSE_MAX_CELLS) return HS_E_ARG;
+    HS_TRY(dense_alloc(D, (n + DB - 1) / DB * DB, f32));
+    (*D)->factorizations += 1;
+    return f32 ? dense_factor_csr_t<float>(c, *D, n, rp, ci, vd, dd) : dense_factor_csr_t<double>(c, *D, n, rp, ci, vd, dd);
+}
+
+int dense_solve_csr(hs_ctx* c, DenseCoarse* D, int n, const int* rp, const int* ci, const double* vd,
+                    const double* dd, const void* b, void* x, bool f32, int refine, const int* skip) {
+    if (!D) return HS_E_STATE;
+    return f32 ? dense_solve_csr_t<float>(c, D, n, rp, ci, vd, dd, (const float*)b, (float*)x, refine, skip)
+               : dense_solve_csr_t<double>(c, D, n, rp, ci, vd, dd, (const double*)b, (double*)x, refine, skip);
+}
+
+void dense_release(DenseCoarse** Dp) {
+    DenseCoarse* D = *Dp;
+    if (!D) return;
+    void* a[6] = {D->M, D->C, D->P, D->r, D->z, D->xf};
+    for (void* p : a) if (p) (void)hipFree(p);
+    delete D;
+    *Dp = nullptr;
+}
 
 int dense_max_cells() { return DENSE_MAX_CELLS; }
 
@@ -424,11 +554,4 @@ int dense_solve(hs_ctx* c, const MGLevel* m, bool f32) {
     return f32 ? dense_solve_t<float>(c, m) : dense_solve_t<double>(c, m);
 }
 
-void dense_free(hs_ctx* c) {
-    DenseCoarse* D = c->dense;
-    if (!D) return;
-    void* a[6] = {D->M, D->C, D->P, D->r, D->z, D->xf};
-    for (void* p : a) if (p) (void)hipFree(p);
-    delete D;
-    c->dense = nullptr;
-}
+void dense_free(hs_ctx* c) { dense_release(&c->dense); }

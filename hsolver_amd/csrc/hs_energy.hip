@@ -31,6 +31,7 @@
 #include <cmath>
 
 #include "hs_device.h"
+#include "hs_eop.h"
 
 using hsd::BT;
 using hsd::BX;
@@ -40,63 +41,7 @@ using hsd::FLAT_BLOCKS;
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ T pos0(T a) {
-    return a < T(0) ? T(0) : a;
-}
-
-template <typename T>
-struct EL {
-    int nx, ny, nz;
-    long sy, sz;
-    const T *dx, *dy, *dz, *fx, *fy, *fz;
-    T *x, *b;
-};
-
-template <typename T>
-EL<T> el(const ELevel* m) {
-    return EL<T>{m->nx, m->ny, m->nz, m->sy, m->sz, (const T*)m->dx, (const T*)m->dy, (const T*)m->dz,
-                 (const T*)m->fx, (const T*)m->fy, (const T*)m->fz, (T*)m->x, (T*)m->b};
-}
-
-struct EOp {
-    double *dx, *dy, *dz, *fx, *fy, *fz;
-};
-
-// Row P: off-diagonal magnitudes of A (tr = 0) or A^T (tr = 1) and the diagonal of A.
-template <typename T>
-struct Row {
-    T w, e, s, n, b, t, p;
-};
-
-template <typename T>
-__device__ __forceinline__ Row<T> row(const T* dx, const T* dy, const T* dz, const T* fx, const T* fy, const T* fz,
-                                      long c, long sy, long sz, int tr) {
-    const T dW = dx[c], dE = dx[c + 1], dS = dy[c], dN = dy[c + sy], dB = dz[c], dT = dz[c + sz];
-    const T fW = fx[c], fE = fx[c + 1], fS = fy[c], fN = fy[c + sy], fB = fz[c], fT = fz[c + sz];
-    const T aw = dW + pos0(fW), ae = dE + pos0(-fE), as = dS + pos0(fS), an = dN + pos0(-fN);
-    const T ab = dB + pos0(fB), at = dT + pos0(-fT);
-    Row<T> r;
-    r.p = ae + aw + an + as + ab + at;
-    if (!tr) {
-        r.w = aw; r.e = ae; r.s = as; r.n = an; r.b = ab; r.t = at;
-    } else {
-        r.w = dW + pos0(-fW); r.e = dE + pos0(fE); r.s = dS + pos0(-fS);
-        r.n = dN + pos0(fN); r.b = dB + pos0(-fB); r.t = dT + pos0(fT);
-    }
-    return r;
-}
-
-template <typename T>
-__device__ __forceinline__ Row<T> lrow(const EL<T>& L, long c, int tr) {
-    return row<T>(L.dx, L.dy, L.dz, L.fx, L.fy, L.fz, c, L.sy, L.sz, tr);
-}
-
-template <typename T, typename TX>
-__device__ __forceinline__ T offs(const Row<T>& r, const TX* x, long c, long sy, long sz) {
-    return r.w * (T)x[c - 1] + r.e * (T)x[c + 1] + r.s * (T)x[c - sy] + r.n * (T)x[c + sy] + r.b * (T)x[c - sz] +
-           r.t * (T)x[c + sz];
-}
+using namespace heo;
 
 // ------------------------------------------------------------ assembly ---
 // Face data of the fine operator and the right-hand side, from k, q, the face
@@ -800,7 +745,10 @@ int e_alloc_levels(hs_ctx* c) {
     EnergyState* E = c->en;
     const Dims& d = c->d;
     int nx = d.nx, ny = d.ny, nz = d.nz;
-    const int maxl = std::min(E_MAX_LEVELS, c->prm.mg_levels > 0 ? c->prm.mg_levels : E_MAX_LEVELS);
+    // the aggregation multigrid builds its own levels below the finest one
+    const int maxl = c->prm.e_precond == HS_PRECOND_AMG
+                         ? 1
+                         : std::min(E_MAX_LEVELS, c->prm.mg_levels > 0 ? c->prm.mg_levels : E_MAX_LEVELS);
     for (int l = 0; l < maxl; ++l) {
         ELevel* m = new ELevel();
         m->nx = nx; m->ny = ny; m->nz = nz;
@@ -845,6 +793,7 @@ template <typename T>
 int e_build(hs_ctx* c) {
     EnergyState* E = c->en;
     const Dims& d = c->d;
+    E->build_id += 1;
     k_e_assemble<<<cells_grid(d.nx, d.ny, d.nz), 64, 0, c->stream>>>(d, c->g, c->k, c->q, c->u, c->v, c->w, c->tf,
                                                                      c->prm.rho, c->prm.outflow, eop(E), E->rhs);
     HS_CHECK(hipGetLastError());
@@ -924,6 +873,12 @@ template <typename T>
 int e_vcycle(hs_ctx* c, int tr) {
     EnergyState* E = c->en;
     const int* skip = &E->ctl->skip;
+    if (c->prm.e_precond == HS_PRECOND_AMG) {
+        // red-black sweeps on the finest level, the aggregated levels below it (hs_amg.hip)
+        HS_TRY(e_smooth<T>(c, E->lv[0], tr, c->prm.amg_sweeps0, true, skip, true));
+        HS_TRY(amg_coarse_e(c, E->lv[0], tr, skip));
+        return e_smooth<T>(c, E->lv[0], tr, c->prm.amg_sweeps0, false, skip, true);
+    }
     const int nl = (int)E->lv.size();
     int t = 0;
     while (t < nl && (long)E->lv[t]->nx * E->lv[t]->ny * E->lv[t]->nz > E_TAIL_MAX_CELLS) ++t;
@@ -990,6 +945,7 @@ int e_bicgstab(hs_ctx* c, const double* b, int tr, double tol, int maxit, hs_sol
     h->maxit = maxit;
     HS_CHECK(hipMemcpyAsync(&E->ctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&E->ctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (c->prm.e_precond == HS_PRECOND_AMG) HS_TRY(amg_setup_e(c, E->lv[0], tr));
     T* b0 = (T*)E->lv[0]->b;
     // the level-0 iterate (the V-cycle's output) moves between two buffers with the fused
     // sweeps: read its current pointer at every use
@@ -1133,6 +1089,7 @@ int energy_apply(hs_ctx* c, int transpose, const double* x, double* y) {
 void energy_free(hs_ctx* c) {
     EnergyState* E = c->en;
     if (!E) return;
+    amg_free(&E->amg);
     for (ELevel* m : E->lv) {
         if (m->own_faces) {
             void* f[6] = {m->dx, m->dy, m->dz, m->fx, m->fy, m->fz};

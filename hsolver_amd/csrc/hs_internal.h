@@ -134,6 +134,8 @@ struct ELevel {
     bool own_faces;    // false when the faces alias the FP64 fine operator
 };
 
+struct AmgHier;
+
 struct EnergyState {
     bool f32 = false;
     std::vector<ELevel*> lv;
@@ -147,6 +149,8 @@ struct EnergyState {
     BiCtl* ctl = nullptr;
     BiCtl* h_ctl = nullptr;                    // pinned mirror
     int last_iters = 0;
+    long build_id = 0;                         // operator builds so far
+    AmgHier* amg = nullptr;                    // aggregation multigrid (e_precond = HS_PRECOND_AMG)
 };
 
 // Dense inverse of the coarsest multigrid level (hs_dense.hip), in the scalar of the
@@ -162,6 +166,40 @@ struct DenseCoarse {
     long builds = 0, factorizations = 0;
     long since = 0;                // hierarchy builds since the last factorization
     int iters_ref = -1;            // FCG iterations of the first solve after it (-1: none yet)
+};
+
+// Aggregation multigrid (hs_amg.hip): the finest level is a structured 7-point operator
+// (the pressure multigrid's level 0 or the energy hierarchy's level 0); every coarser
+// level is a CSR matrix over aggregates of the level above it.
+struct AmgLevel {
+    long n = 0, nnz = 0;            // rows, stored off-diagonal entries
+    int *rp = nullptr, *ci = nullptr;
+    double *vd = nullptr, *dd = nullptr;   // entries and diagonal in FP64 (setup, refinement)
+    void *val = nullptr, *dg = nullptr;    // the same in the cycle's scalar (alias when FP64)
+    void *x = nullptr, *x2 = nullptr, *b = nullptr;
+    int* agg = nullptr;             // rows of the level above -> rows of this level (-1: none)
+    int* memb = nullptr;            // 4 per row: rows of the level above in this aggregate (-1: none)
+    // the first pairwise pass's intermediate matrix (kept for the numeric rebuild)
+    long ni = 0, nnzi = 0;
+    int *rpi = nullptr, *cii = nullptr, *membi = nullptr, *aggi = nullptr;
+    double *vdi = nullptr, *ddi = nullptr;
+    int* membf = nullptr;           // 2 per intermediate row: rows of the level above
+    int* aggb = nullptr;            // intermediate rows -> rows of this level
+};
+
+struct AmgHier {
+    bool f32 = false;
+    int kind = -1;                  // 0: p' operator, 1: energy A, 2: energy A^T
+    long n0 = 0;                    // rows of the structured finest level (padded cells)
+    std::vector<AmgLevel*> lv;      // aggregated levels, coarsest last
+    DenseCoarse* dense = nullptr;   // inverse of the coarsest level when it is small enough
+    int* scratch = nullptr;         // matching / numbering work (ints)
+    long scratch_n = 0;
+    int* dflag = nullptr;           // device: [0] row-width overflow, [1] scan total
+    int* h_flag = nullptr;          // pinned mirror
+    long builds = 0, aggregations = 0, since = 0;
+    int iters_ref = -1;             // p' FCG iterations of the first solve after an aggregation
+    long built_id = -1;             // energy: the operator build the hierarchy was made from
 };
 
 struct hs_ctx {
@@ -220,6 +258,8 @@ struct hs_ctx {
     EnergyState* en = nullptr;
     // dense coarsest-level inverse (created on demand)
     DenseCoarse* dense = nullptr;
+    // aggregation multigrid of the p' operator (HS_PRECOND_AMG, created on demand)
+    AmgHier* amgp = nullptr;
     int last_iters = 0;
 
     // live per-kernel timing: HIP events around every launch of one chosen kernel
@@ -296,6 +336,24 @@ int dense_max_cells();
 int dense_setup(hs_ctx* c, const MGLevel* m, bool f32);
 int dense_solve(hs_ctx* c, const MGLevel* m, bool f32);
 void dense_free(hs_ctx* c);
+// dense inverse of a CSR level (order n <= dense_max_cells()): factor, and the per-cycle
+// apply x = A^-1 b (b, x in the scalar of the inverse) with `refine` FP64 refinement steps
+// against the level's FP64 entries; kernels return at once while *skip != 0
+int dense_factor_csr(hs_ctx* c, DenseCoarse** D, int n, const int* rp, const int* ci, const double* vd,
+                     const double* dd, bool f32);
+int dense_solve_csr(hs_ctx* c, DenseCoarse* D, int n, const int* rp, const int* ci, const double* vd,
+                    const double* dd, const void* b, void* x, bool f32, int refine, const int* skip);
+void dense_release(DenseCoarse** D);
+// aggregation multigrid (hs_amg.hip).  The structured finest level is the pressure
+// multigrid's level 0 (amg_*_p) or the energy hierarchy's level 0 with A (tr = 0) or A^T
+// (tr = 1) (amg_*_e).  amg_setup_* builds or refreshes the aggregated levels from it;
+// amg_coarse_* runs the part of one cycle below the finest level: restrict b - A x of
+// the finest level, cycle the aggregated levels, add the scaled correction to x.
+int amg_setup_p(hs_ctx* c);
+int amg_coarse_p(hs_ctx* c, const int* skip);
+int amg_setup_e(hs_ctx* c, ELevel* l0, int tr);
+int amg_coarse_e(hs_ctx* c, ELevel* l0, int tr, const int* skip);
+void amg_free(AmgHier** H);
 // energy equation on the frozen flow (hs_energy.hip); vectors in the C layout are device pointers
 int energy_solve(hs_ctx* c, double tol, int maxit, hs_solve_info* info);
 int energy_adjoint(hs_ctx* c, const double* dJdT, double* dJdq, double* dJdk, double tol, int maxit,

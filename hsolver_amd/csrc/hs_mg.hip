@@ -777,7 +777,7 @@ int alloc_level(MGLevel* m) {
 // zx: the level's iterate is zero on entry and the first fused sweep does not read it
 // (the callers then skip writing those zeros)
 template <typename T>
-int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
+int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false, float omega_set = 0.0f) {
     Lv<T> L = lv<T>(m);
     const long tiles = (long)((L.nx + BX - 1) / BX) * ((L.ny + BY - 1) / BY);
     const int kc = hsd::kchunk(L.nz, tiles);
@@ -790,7 +790,7 @@ int smooth(hs_ctx* c, MGLevel* m, int sweeps, bool forward, bool zx = false) {
     // in-place colour passes run instead
     const int rows = c->prm.mg_sweep_rows;
     // over-relaxation: the finest level's, or that of the levels below it
-    const T omega = (T)(m->level == 0 ? c->prm.mg_omega : c->prm.mg_omega_coarse);
+    const T omega = (T)(omega_set > 0.0f ? omega_set : m->level == 0 ? c->prm.mg_omega : c->prm.mg_omega_coarse);
     if (sweeps > 0 && rows >= 0) {
         for (; sweeps > 0; --sweeps) {
             const T* src = (T*)m->x;
@@ -875,8 +875,20 @@ inline bool cycle_zero_guess(const hs_ctx* c) {
     return cycle_end(c, &dense) > 0 && zero_guess(c, 0);
 }
 
+// Aggregation multigrid cycle: red-black sweeps on the finest level (the level-0 iterate
+// is zero on entry), the aggregated levels below it (hs_amg.hip), sweeps in the
+// opposite colour order
+template <typename T>
+int amg_vcycle(hs_ctx* c) {
+    MGLevel* m = c->mg[0];
+    HS_TRY(smooth<T>(c, m, c->prm.amg_sweeps0, true, false, c->prm.amg_omega));
+    HS_TRY(amg_coarse_p(c, &c->sctl->done));
+    return smooth<T>(c, m, c->prm.amg_sweeps0, false, false, c->prm.amg_omega);
+}
+
 template <typename T>
 int vcycle(hs_ctx* c) {
+    if (c->prm.precond == HS_PRECOND_AMG) return amg_vcycle<T>(c);
     bool dense;
     const int t = cycle_end(c, &dense);
     return cycle<T>(c, 0, t, dense, cycle_zero_guess(c));
@@ -898,6 +910,8 @@ int build_hierarchy(hs_ctx* c) {
             m->act = nullptr;
             c->mg.push_back(m);
             HS_TRY(alloc_level<T>(m));
+            // the aggregation multigrid builds its own levels below the finest one
+            if (c->prm.precond == HS_PRECOND_AMG) break;
             if (std::min(nx, std::min(ny, nz)) <= 4) break;
             // with the dense coarse solve the hierarchy ends at the first small enough level
             if (c->prm.mg_dense_coarse > 0 && (long)nx * ny * nz <= dense_max_cells()) break;
@@ -908,6 +922,7 @@ int build_hierarchy(hs_ctx* c) {
     k_mg_fine<T><<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->ax, c->ay, c->az, L0,
                                                                           c->mg[0]->act);
     HS_CHECK(hipGetLastError());
+    if (c->prm.precond == HS_PRECOND_AMG) return amg_setup_p(c);
     for (size_t l = 0; l + 1 < c->mg.size(); ++l) {
         Lv<T> F = lv<T>(c->mg[l]), C = lv<T>(c->mg[l + 1]);
         // ground conductance: the Dirichlet point sits half a finest-level cell outside the
@@ -1001,6 +1016,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     }
     c->last_iters = h->iters;
     if (c->dense && c->dense->iters_ref < 0) c->dense->iters_ref = h->iters;
+    if (c->amgp && c->amgp->iters_ref < 0) c->amgp->iters_ref = h->iters;
     if (st) {
         st->p_iters = h->iters;
         st->p_converged = h->converged;
@@ -1027,6 +1043,7 @@ int fcg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_res
 
 void gmg_free(hs_ctx* c) {
     dense_free(c);
+    amg_free(&c->amgp);
     for (MGLevel* m : c->mg) {
         void* arrs[8] = {m->cw, m->cs, m->cb, m->g, m->x, m->b, m->x2, m->act};
         for (void* a : arrs) if (a) (void)hipFree(a);

@@ -76,6 +76,8 @@ extern "C" int hs_pressure_solve_static(hs_ctx* c, const double* b_dev, double* 
         return HS_E_ARG;
     }
     if (!c->have_matrix) return HS_E_STATE;
+    // the aggregation multigrid allocates when it aggregates again: not available here
+    if (c->prm.precond == HS_PRECOND_AMG) return HS_E_ARG;
     // the multigrid solve allocates its hierarchy (and dense coarse inverse) on first use
     if (c->prm.precond == HS_PRECOND_GMG && (c->mg.empty() || (c->prm.mg_dense_coarse > 0 && !c->dense)))
         return HS_E_STATE;

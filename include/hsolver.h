@@ -64,6 +64,9 @@ extern "C" {
 /* pressure-correction solver */
 #define HS_PRECOND_NONE 0   /* CG with identity preconditioner: HYPRE ParCSR PCG as coded */
 #define HS_PRECOND_GMG  1   /* flexible CG preconditioned by geometric multigrid */
+#define HS_PRECOND_AMG  2   /* flexible CG preconditioned by aggregation (algebraic) multigrid:
+                               the slot the reference reserves for BoomerAMG
+                               (fluid_solver.h:1775-1787, 1876-1888) */
 
 #define HS_PREC_FP64    0
 #define HS_PREC_MIXED   1   /* FP32 multigrid inside FP64 flexible CG */
@@ -141,6 +144,24 @@ typedef struct {
                                          solutions (2 x1 - x2, 3 x1 - 3 x2 + x3; fewer while fewer
                                          are held).  The stop rule stays ||r|| / ||b|| < p_tol, and a
                                          start no better than 0 is replaced by 0 */
+    int32_t e_precond;                /* preconditioner of the energy solves (hs_energy_solve,
+                                         hs_energy_solve_rhs, hs_adjoint_energy): HS_PRECOND_GMG
+                                         (default; 0 means the same) or HS_PRECOND_AMG */
+    /* aggregation multigrid (precond or e_precond = HS_PRECOND_AMG): the structured finest
+       level, then levels of aggregates formed by two passes of pairwise matching along
+       strong couplings, Galerkin coarse operators, a dense inverse on the matrix cores once
+       a level has at most 4096 rows (FP64 refinement), V-cycle */
+    int32_t amg_sweeps0;              /* red-black Gauss-Seidel sweeps on the finest level before and
+                                         after the coarse correction (<= 0: 1) */
+    int32_t amg_sweeps;               /* Jacobi sweeps (weight 2/3) before and after on every
+                                         aggregated level (<= 0: 2) */
+    float   amg_theta;                /* strong coupling: the pair's symmetrised coupling over the
+                                         geometric mean of the two diagonals is at least this
+                                         fraction of the row's largest (<= 0: 0.25) */
+    float   amg_alpha;                /* scaling of every prolongated coarse correction (<= 0: 1 for
+                                         the p' solve, 1.6 for the energy solves) */
+    float   amg_omega;                /* over-relaxation of the finest level's sweeps (<= 0: 1) */
+    int32_t amg_pad;
 } hs_params;
 
 typedef struct {
