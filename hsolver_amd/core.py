@@ -238,6 +238,17 @@ class Context:
         self._call("energy_apply", 1 if transpose else 0, xa.ctypes.data, y.ctypes.data, abi.HS_MEM_HOST)
         return y.reshape(self.nz, self.ny, self.nx)
 
+    def amg_levels(self, which: int = 0) -> list:
+        """(rows, off-diagonal entries) of each aggregated level: which = 0 the p' hierarchy,
+        1 the energy hierarchy."""
+        fn = self.lib.hs_amg_levels
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+        buf = np.zeros(2 * 32, dtype=np.int64)
+        n = fn(self.h, which, buf.ctypes.data, 32)
+        if n < 0:
+            raise HSError(n, "hs_amg_levels", "")
+        return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(n)]
+
     def cells(self, name: str) -> np.ndarray:
         which = abi.ARRAY_NAMES.index(name)
         out = np.empty(self.nx * self.ny * self.nz)

@@ -8,9 +8,10 @@
 //                  hold (the pressure multigrid's level 0, or the energy hierarchy's
 //                  level 0 as A or A^T), smoothed by their red-black Gauss-Seidel sweeps;
 //   aggregation    two passes of pairwise matching per level, so an aggregate holds up
-//                  to four rows of the level above.  A pass matches by handshakes: every
-//                  unmatched row proposes to its strongest unmatched neighbour and mutual
-//                  proposals become pairs (eight rounds; rows left over stay single).
+//                  to four rows of the level above.  A pass runs eight handshake rounds
+//                  in which hashed colours split the unmatched rows into proposers and
+//                  acceptors: a proposer asks its strongest unmatched acceptor neighbour,
+//                  an acceptor takes the strongest proposer (rows left over stay single).
 //                  Strength of the pair (i, j): the symmetrised coupling -(a_ij + a_ji)/2
 //                  over sqrt(a_ii a_jj), and only couplings of at least theta times the
 //                  row's largest are candidates.  Scaling by the diagonals keeps large
@@ -50,6 +51,7 @@ namespace {
 
 constexpr int ROUNDS = 8;          // handshake rounds of one pairwise pass
 constexpr int MAXW = 128;          // distinct coarse columns of one Galerkin row
+constexpr int SL = 64;             // rows per slice of the sliced ELL format
 constexpr int GAL_T = 64;          // threads of a Galerkin block (one row each)
 constexpr int AMG_MAX_LEVELS = 24;
 constexpr int SCAN_T = 256, SCAN_ITEMS = 4, SCAN_BLOCK = SCAN_T * SCAN_ITEMS;
@@ -97,30 +99,51 @@ struct EView {   // energy operator A (tr = 0) or A^T (tr = 1) on the energy lev
 };
 
 // ------------------------------------------------------------ pairwise matching ---
-// proposal of row c: its strongest unmatched neighbour among the strong ones, -1 if none
+// One round: every unmatched row gets a colour from a hash of (row, round).  A blue row
+// proposes to its strongest unmatched red neighbour among its strong ones; a red row
+// accepts the strongest blue row that proposed to it, and writes both ends of the pair.
+// A blue row proposes to one red row only, so no two red rows write the same entry.
+// (Proposals between rows of one colour class would follow the strength gradient of a
+// stretched grid row by row and leave long proposal chains unmatched.)
+__device__ __forceinline__ bool is_blue(long i, int round) {
+    unsigned long long x = (unsigned long long)i * 0x9E3779B97F4A7C15ull ^ (0xD1B54A32D192ED03ull * (unsigned)(round + 1));
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 29;
+    return (x >> 40) & 1ull;
+}
+
+// strengths of row c's six couplings (0 where the neighbour is not an unknown) and their max
 template <typename V>
-__global__ __launch_bounds__(FLAT) void k_propose0(V v, long n, const int* mate, int* prop, float theta) {
+__device__ __forceinline__ double strengths0(const V& v, long c, double s[6]) {
+    double a[6];
+    v.coup(c, a);
+    const double dc = v.diag(c);
+    double smax = 0.0;
+    for (int d = 0; d < 6; ++d) {
+        s[d] = 0.0;
+        const long nb = c + v.off(d);
+        if (!v.active(nb)) continue;
+        double an[6];
+        v.coup(nb, an);
+        const double sym = 0.5 * (a[d] + an[d ^ 1]), dn = v.diag(nb);
+        if (sym > 0.0 && dc > 0.0 && dn > 0.0) s[d] = sym / sqrt(dc * dn);
+        smax = fmax(smax, s[d]);
+    }
+    return smax;
+}
+
+template <typename V>
+__global__ __launch_bounds__(FLAT) void k_propose0(V v, long n, const int* mate, int* prop, float theta, int round) {
     G_STRIDE(c, n) {
         int best = -1;
-        if (v.active(c) && mate[c] < 0) {
-            double a[6], s[6];
-            v.coup(c, a);
-            const double dc = v.diag(c);
-            double smax = 0.0;
-            for (int d = 0; d < 6; ++d) {
-                s[d] = 0.0;
-                const long nb = c + v.off(d);
-                if (!v.active(nb)) continue;
-                double an[6];
-                v.coup(nb, an);
-                const double sym = 0.5 * (a[d] + an[d ^ 1]), dn = v.diag(nb);
-                if (sym > 0.0 && dc > 0.0 && dn > 0.0) s[d] = sym / sqrt(dc * dn);
-                smax = fmax(smax, s[d]);
-            }
+        if (v.active(c) && mate[c] < 0 && is_blue(c, round)) {
+            double s[6];
+            const double smax = strengths0(v, c, s);
             double bs = 0.0;
             for (int d = 0; d < 6; ++d) {
                 const long nb = c + v.off(d);
-                if (s[d] > 0.0 && s[d] >= (double)theta * smax && mate[nb] < 0 &&
+                if (s[d] > 0.0 && s[d] >= (double)theta * smax && mate[nb] < 0 && !is_blue(nb, round) &&
                     (s[d] > bs || (s[d] == bs && nb < best))) {
                     bs = s[d];
                     best = (int)nb;
@@ -128,6 +151,28 @@ __global__ __launch_bounds__(FLAT) void k_propose0(V v, long n, const int* mate,
             }
         }
         prop[c] = best;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(FLAT) void k_respond0(V v, long n, int* mate, const int* prop, int round) {
+    G_STRIDE(j, n) {
+        if (!v.active(j) || mate[j] >= 0 || is_blue(j, round)) continue;
+        double s[6];
+        strengths0(v, j, s);
+        int best = -1;
+        double bs = 0.0;
+        for (int d = 0; d < 6; ++d) {
+            const long i = j + v.off(d);
+            if (s[d] > 0.0 && prop[i] == (int)j && (s[d] > bs || (s[d] == bs && i < best))) {
+                bs = s[d];
+                best = (int)i;
+            }
+        }
+        if (best >= 0) {
+            mate[j] = best;
+            mate[best] = (int)j;
+        }
     }
 }
 
@@ -142,16 +187,17 @@ __device__ __forceinline__ double csr_strength(int i, int t, const int* rp, cons
 }
 
 __global__ __launch_bounds__(FLAT) void k_propose_csr(long n, const int* rp, const int* ci, const double* vd,
-                                                      const double* dd, const int* mate, int* prop, float theta) {
+                                                      const double* dd, const int* mate, int* prop, float theta,
+                                                      int round) {
     G_STRIDE(i, n) {
         int best = -1;
-        if (mate[i] < 0) {
+        if (mate[i] < 0 && is_blue(i, round)) {
             double smax = 0.0;
             for (int t = rp[i]; t < rp[i + 1]; ++t) smax = fmax(smax, csr_strength((int)i, t, rp, ci, vd, dd));
             double bs = 0.0;
             for (int t = rp[i]; t < rp[i + 1]; ++t) {
                 const int j = ci[t];
-                if (mate[j] >= 0) continue;
+                if (mate[j] >= 0 || is_blue(j, round)) continue;
                 const double s = csr_strength((int)i, t, rp, ci, vd, dd);
                 if (s > 0.0 && s >= (double)theta * smax && (s > bs || (s == bs && j < best))) {
                     bs = s;
@@ -163,11 +209,26 @@ __global__ __launch_bounds__(FLAT) void k_propose_csr(long n, const int* rp, con
     }
 }
 
-// mutual proposals become pairs (each row writes only its own entry)
-__global__ __launch_bounds__(FLAT) void k_accept(long n, const int* prop, int* mate) {
-    G_STRIDE(i, n) {
-        const int p = prop[i];
-        if (p >= 0 && mate[i] < 0 && prop[p] == (int)i) mate[i] = p;
+// the matrix pattern is symmetric, so every row that proposed to j is in j's row
+__global__ __launch_bounds__(FLAT) void k_respond_csr(long n, const int* rp, const int* ci, const double* vd,
+                                                      const double* dd, int* mate, const int* prop, int round) {
+    G_STRIDE(j, n) {
+        if (mate[j] >= 0 || is_blue(j, round)) continue;
+        int best = -1;
+        double bs = 0.0;
+        for (int t = rp[j]; t < rp[j + 1]; ++t) {
+            const int i = ci[t];
+            if (prop[i] != (int)j) continue;
+            const double s = csr_strength((int)j, t, rp, ci, vd, dd);
+            if (s > 0.0 && (s > bs || (s == bs && i < best))) {
+                bs = s;
+                best = i;
+            }
+        }
+        if (best >= 0) {
+            mate[j] = best;
+            mate[best] = (int)j;
+        }
     }
 }
 
@@ -287,13 +348,14 @@ struct GalOut {
     int* flag;
 };
 
+template <int W>
 __device__ __forceinline__ void gal_add(int (*cols)[GAL_T], double (*vals)[GAL_T], int& m, bool& over, int J,
                                         double e) {
     const int t0 = threadIdx.x;
     int t = 0;
     while (t < m && cols[t][t0] != J) ++t;
     if (t < m) vals[t][t0] += e;
-    else if (m < MAXW) { cols[m][t0] = J; vals[m][t0] = e; ++m; }
+    else if (m < W) { cols[m][t0] = J; vals[m][t0] = e; ++m; }
     else over = true;
 }
 
@@ -313,10 +375,10 @@ __device__ __forceinline__ void gal_store(const GalOut& o, long I, int (*cols)[G
     o.dd[I] = diag;
 }
 
-template <typename V>
+template <typename V, int W>
 __global__ __launch_bounds__(GAL_T) void k_gal0(V v, long nc, const int* memb2, const int* agg, GalOut o) {
-    __shared__ int cols[MAXW][GAL_T];
-    __shared__ double vals[MAXW][GAL_T];
+    __shared__ int cols[W][GAL_T];
+    __shared__ double vals[W][GAL_T];
     const long I = blockIdx.x * (long)GAL_T + threadIdx.x;
     if (I >= nc) return;
     int m = 0;
@@ -333,16 +395,17 @@ __global__ __launch_bounds__(GAL_T) void k_gal0(V v, long nc, const int* memb2, 
             if (!v.active(nb)) continue;
             const int J = agg[nb];
             if (J == (int)I) diag -= a[d];
-            else gal_add(cols, vals, m, over, J, -a[d]);
+            else gal_add<W>(cols, vals, m, over, J, -a[d]);
         }
     }
     gal_store(o, I, cols, vals, m, over, diag);
 }
 
+template <int W>
 __global__ __launch_bounds__(GAL_T) void k_gal_csr(long nc, const int* memb2, const int* agg, const int* frp,
                                                    const int* fci, const double* fvd, const double* fdd, GalOut o) {
-    __shared__ int cols[MAXW][GAL_T];
-    __shared__ double vals[MAXW][GAL_T];
+    __shared__ int cols[W][GAL_T];
+    __shared__ double vals[W][GAL_T];
     const long I = blockIdx.x * (long)GAL_T + threadIdx.x;
     if (I >= nc) return;
     int m = 0;
@@ -355,7 +418,7 @@ __global__ __launch_bounds__(GAL_T) void k_gal_csr(long nc, const int* memb2, co
         for (int t = frp[r]; t < frp[r + 1]; ++t) {
             const int J = agg[fci[t]];
             if (J == (int)I) diag += fvd[t];
-            else gal_add(cols, vals, m, over, J, fvd[t]);
+            else gal_add<W>(cols, vals, m, over, J, fvd[t]);
         }
     }
     gal_store(o, I, cols, vals, m, over, diag);
@@ -374,57 +437,95 @@ __global__ __launch_bounds__(FLAT) void k_jac0(long n, const T* dg, const T* b, 
     G_STRIDE(i, n) x[i] = w * b[i] / dg[i];
 }
 
+// sliced ELL: slice widths (the longest row of each 64) and their entry counts
+__global__ __launch_bounds__(FLAT) void k_sell_width(long n, const int* rp, int* sw, int* cnt) {
+    const long ns = (n + SL - 1) / SL;
+    G_STRIDE(s, ns) {
+        int w = 0;
+        const long i1 = min(n, (s + 1) * SL);
+        for (long i = s * SL; i < i1; ++i) w = max(w, rp[i + 1] - rp[i]);
+        sw[s] = w;
+        cnt[s] = w * SL;
+    }
+}
+
+// entries of row i at so[s] + k SL + (i mod SL); a row shorter than its slice is padded
+// with entries of value 0 on its own column
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_sell_fill(long n, const int* rp, const int* ci, const double* vd, const int* so,
+                                                    const int* sw, int* sc, T* sv, int pattern) {
+    G_STRIDE(i, n) {
+        const long s = i / SL;
+        const long base = so[s] + (i % SL);
+        const int r0 = rp[i], len = rp[i + 1] - r0, w = sw[s];
+        for (int k = 0; k < w; ++k) {
+            if (pattern) sc[base + (long)k * SL] = k < len ? ci[r0 + k] : (int)i;
+            sv[base + (long)k * SL] = k < len ? (T)vd[r0 + k] : T(0);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T sell_offsum(long i, const int* so, const int* sw, const int* sc, const T* sv, const T* x) {
+    const long s = i / SL;
+    const long base = so[s] + (i % SL);
+    const int w = sw[s];
+    T acc = T(0);
+    for (int k = 0; k < w; ++k) acc += sv[base + (long)k * SL] * x[sc[base + (long)k * SL]];
+    return acc;
+}
+
 // xo = x + w (b - A x) / d
 template <typename T>
-__global__ __launch_bounds__(FLAT) void k_jac(long n, const int* rp, const int* ci, const T* val, const T* dg,
-                                              const T* b, const T* x, T* xo, T w, const int* skip) {
+__global__ __launch_bounds__(FLAT) void k_jac(long n, const int* so, const int* sw, const int* sc, const T* sv,
+                                              const T* dg, const T* b, const T* x, T* xo, T w, const int* skip) {
     if (*skip) return;
     G_STRIDE(i, n) {
         const T xi = x[i];
-        T s = b[i] - dg[i] * xi;
-        for (int t = rp[i]; t < rp[i + 1]; ++t) s -= val[t] * x[ci[t]];
+        const T s = b[i] - dg[i] * xi - sell_offsum(i, so, sw, sc, sv, x);
         xo[i] = xi + w * s / dg[i];
     }
 }
 
-// bc[I] = sum over the members r of I of (b - A x)_r
+// r = b - A x
 template <typename T>
-__global__ __launch_bounds__(FLAT) void k_restrict_csr(long nc, const int* memb4, const int* rp, const int* ci,
-                                                       const T* val, const T* dg, const T* b, const T* x, T* bc,
-                                                       const int* skip) {
+__global__ __launch_bounds__(FLAT) void k_resid(long n, const int* so, const int* sw, const int* sc, const T* sv,
+                                                const T* dg, const T* b, const T* x, T* r, const int* skip) {
     if (*skip) return;
-    G_STRIDE(I, nc) {
-        T s = T(0);
-        for (int q = 0; q < 4; ++q) {
-            const int r = memb4[4 * I + q];
-            if (r < 0) continue;
-            T e = b[r] - dg[r] * x[r];
-            for (int t = rp[r]; t < rp[r + 1]; ++t) e -= val[t] * x[ci[t]];
-            s += e;
-        }
-        bc[I] = s;
-    }
+    G_STRIDE(i, n) r[i] = b[i] - dg[i] * x[i] - sell_offsum(i, so, sw, sc, sv, x);
 }
 
+// r = b - A x on the structured finest level (0 on rows that are not unknowns)
 template <typename V, typename T>
-__global__ __launch_bounds__(FLAT) void k_restrict0(V v, long nc, const int* memb4, T* bc, const int* skip) {
+__global__ __launch_bounds__(FLAT) void k_resid0(V v, long n, T* r, const int* skip) {
     if (*skip) return;
     const T* x = v.x();
     const T* b = v.b();
-    G_STRIDE(I, nc) {
-        T s = T(0);
-        for (int q = 0; q < 4; ++q) {
-            const int r = memb4[4 * I + q];
-            if (r < 0) continue;
+    G_STRIDE(c, n) {
+        T e = T(0);
+        if (v.active(c)) {
             double a[6];
-            v.coup(r, a);
-            T e = b[r] - (T)v.diag(r) * x[r];
+            v.coup(c, a);
+            e = b[c] - (T)v.diag(c) * x[c];
             for (int d = 0; d < 6; ++d) {
-                const long nb = r + v.off(d);
+                const long nb = c + v.off(d);
                 if (v.active(nb)) e += (T)a[d] * x[nb];
             }
-            s += e;
         }
+        r[c] = e;
+    }
+}
+
+// bc[I] = sum of r over the members of I
+template <typename T>
+__global__ __launch_bounds__(FLAT) void k_gather4(long nc, const int* memb4, const T* r, T* bc, const int* skip) {
+    if (*skip) return;
+    G_STRIDE(I, nc) {
+        const int4 m = reinterpret_cast<const int4*>(memb4)[I];
+        T s = r[m.x];
+        if (m.y >= 0) s += r[m.y];
+        if (m.z >= 0) s += r[m.z];
+        if (m.w >= 0) s += r[m.w];
         bc[I] = s;
     }
 }
@@ -452,10 +553,9 @@ int amalloc(void** p, size_t bytes) {
 }
 
 void level_free(AmgLevel* L) {
-    void* a[] = {L->rp, L->ci, L->vd, L->dd, L->x, L->x2, L->b, L->agg, L->memb, L->rpi, L->cii, L->membi, L->aggi,
-                 L->vdi, L->ddi, L->membf, L->aggb};
+    void* a[] = {L->rp, L->ci, L->vd, L->dd, L->x, L->x2, L->b, L->r, L->agg, L->memb, L->rpi, L->cii, L->membi,
+                 L->aggi, L->vdi, L->ddi, L->membf, L->aggb, L->so, L->sw, L->sc, L->sv};
     for (void* p : a) if (p) (void)hipFree(p);
-    if (L->val && L->val != (void*)L->vd) (void)hipFree(L->val);
     if (L->dg && L->dg != (void*)L->dd) (void)hipFree(L->dg);
     delete L;
 }
@@ -499,8 +599,8 @@ int pair_pass0(hs_ctx* c, AmgHier* H, const V& v, long n, float theta, long* nag
     const int nb = blocks_for(n);
     HS_CHECK(hipMemsetAsync(mate, 0xff, n * sizeof(int), c->stream));
     for (int r = 0; r < ROUNDS; ++r) {
-        k_propose0<V><<<nb, FLAT, 0, c->stream>>>(v, n, mate, prop, theta);
-        k_accept<<<nb, FLAT, 0, c->stream>>>(n, prop, mate);
+        k_propose0<V><<<nb, FLAT, 0, c->stream>>>(v, n, mate, prop, theta, r);
+        k_respond0<V><<<nb, FLAT, 0, c->stream>>>(v, n, mate, prop, r);
     }
     k_lead0<V><<<nb, FLAT, 0, c->stream>>>(v, n, mate, flag);
     HS_CHECK(hipGetLastError());
@@ -520,8 +620,8 @@ int pair_pass_csr(hs_ctx* c, AmgHier* H, long n, const int* rp, const int* ci, c
     const int nb = blocks_for(n);
     HS_CHECK(hipMemsetAsync(mate, 0xff, n * sizeof(int), c->stream));
     for (int r = 0; r < ROUNDS; ++r) {
-        k_propose_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, mate, prop, theta);
-        k_accept<<<nb, FLAT, 0, c->stream>>>(n, prop, mate);
+        k_propose_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, mate, prop, theta, r);
+        k_respond_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, mate, prop, r);
     }
     k_lead_csr<<<nb, FLAT, 0, c->stream>>>(n, mate, flag);
     HS_CHECK(hipGetLastError());
@@ -567,18 +667,33 @@ int galerkin(hs_ctx* c, AmgHier* H, long nc, F launch, int** rp, int** ci, doubl
 
 inline int gal_blocks(long nc) { return (int)((nc + GAL_T - 1) / GAL_T); }
 
-// the cycle's copies of a level's entries (aliases of the FP64 ones in an FP64 hierarchy)
+// the cycle's copies of a level's entries: the sliced ELL form (pattern built on the first
+// call, values refilled on later ones) and the diagonal (an alias in an FP64 hierarchy)
 template <typename T>
-int narrow_level(hs_ctx* c, AmgLevel* L) {
-    if (sizeof(T) == sizeof(double)) {
-        L->val = L->vd;
-        L->dg = L->dd;
-        return HS_OK;
+int sell_level(hs_ctx* c, AmgHier* H, AmgLevel* L) {
+    const long n = L->n, ns = (n + SL - 1) / SL;
+    const bool first = !L->so;
+    if (first) {
+        HS_TRY(ensure_scratch(H, ns + 1));
+        int* cnt = H->scratch;
+        int* bsum = H->scratch + 2 * (ns + 1);
+        HS_TRY(amalloc((void**)&L->sw, std::max(1L, ns) * sizeof(int)));
+        HS_TRY(amalloc((void**)&L->so, (ns + 1) * sizeof(int)));
+        k_sell_width<<<blocks_for(ns), FLAT, 0, c->stream>>>(n, L->rp, L->sw, cnt);
+        HS_CHECK(hipGetLastError());
+        HS_CHECK(hipMemsetAsync(cnt + ns, 0, sizeof(int), c->stream));
+        HS_TRY(scan(c, H, ns + 1, cnt, L->so, bsum, &L->nsell));
+        HS_TRY(amalloc((void**)&L->sc, std::max(1L, L->nsell) * sizeof(int)));
+        HS_TRY(amalloc(&L->sv, std::max(1L, L->nsell) * sizeof(T)));
     }
-    if (!L->val) HS_TRY(amalloc(&L->val, std::max(1L, L->nnz) * sizeof(T)));
-    if (!L->dg) HS_TRY(amalloc(&L->dg, std::max(1L, L->n) * sizeof(T)));
-    k_narrow<T><<<blocks_for(L->nnz), FLAT, 0, c->stream>>>(L->nnz, L->vd, (T*)L->val);
-    k_narrow<T><<<blocks_for(L->n), FLAT, 0, c->stream>>>(L->n, L->dd, (T*)L->dg);
+    k_sell_fill<T><<<blocks_for(n), FLAT, 0, c->stream>>>(n, L->rp, L->ci, L->vd, L->so, L->sw, L->sc, (T*)L->sv,
+                                                          first ? 1 : 0);
+    if (sizeof(T) == sizeof(double)) {
+        L->dg = L->dd;
+    } else {
+        if (!L->dg) HS_TRY(amalloc(&L->dg, std::max(1L, n) * sizeof(T)));
+        k_narrow<T><<<blocks_for(n), FLAT, 0, c->stream>>>(n, L->dd, (T*)L->dg);
+    }
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
@@ -589,7 +704,30 @@ int level_vectors(AmgLevel* L) {
     HS_TRY(amalloc(&L->x, b));
     HS_TRY(amalloc(&L->x2, b));
     HS_TRY(amalloc(&L->b, b));
+    HS_TRY(amalloc(&L->r, b));
     return HS_OK;
+}
+
+// Galerkin kernels for rows of at most W distinct columns (W from the bound of the row
+// lengths: twice the longest row of the level above, two members per aggregate)
+inline int width_class(int bound) { return bound <= 16 ? 16 : bound <= 32 ? 32 : bound <= 64 ? 64 : MAXW; }
+
+template <typename V>
+void launch_gal0(hs_ctx* c, const V& v, long nc, const int* memb2, const int* agg, const GalOut& o, int W) {
+    const int g = gal_blocks(nc);
+    if (W <= 16) k_gal0<V, 16><<<g, GAL_T, 0, c->stream>>>(v, nc, memb2, agg, o);
+    else if (W <= 32) k_gal0<V, 32><<<g, GAL_T, 0, c->stream>>>(v, nc, memb2, agg, o);
+    else if (W <= 64) k_gal0<V, 64><<<g, GAL_T, 0, c->stream>>>(v, nc, memb2, agg, o);
+    else k_gal0<V, MAXW><<<g, GAL_T, 0, c->stream>>>(v, nc, memb2, agg, o);
+}
+
+void launch_gal_csr(hs_ctx* c, long nc, const int* memb2, const int* agg, const int* frp, const int* fci,
+                    const double* fvd, const double* fdd, const GalOut& o, int W) {
+    const int g = gal_blocks(nc);
+    if (W <= 16) k_gal_csr<16><<<g, GAL_T, 0, c->stream>>>(nc, memb2, agg, frp, fci, fvd, fdd, o);
+    else if (W <= 32) k_gal_csr<32><<<g, GAL_T, 0, c->stream>>>(nc, memb2, agg, frp, fci, fvd, fdd, o);
+    else if (W <= 64) k_gal_csr<64><<<g, GAL_T, 0, c->stream>>>(nc, memb2, agg, frp, fci, fvd, fdd, o);
+    else k_gal_csr<MAXW><<<g, GAL_T, 0, c->stream>>>(nc, memb2, agg, frp, fci, fvd, fdd, o);
 }
 
 // Galerkin matrix of the intermediate (pass A) aggregates of L from the level above:
@@ -599,13 +737,14 @@ int refill_intermediate(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine
     const long ni = L->ni;
     const int* membf = L->membf;
     const int* aggi = L->aggi;
+    L->wbi = std::min(MAXW, 2 * (fine ? fine->wb : 6));
+    const int W = width_class(L->wbi);
     if (fine)
         return galerkin(c, H, ni, [&](GalOut o) {
-            k_gal_csr<<<gal_blocks(ni), GAL_T, 0, c->stream>>>(ni, membf, aggi, fine->rp, fine->ci, fine->vd, fine->dd, o);
+            launch_gal_csr(c, ni, membf, aggi, fine->rp, fine->ci, fine->vd, fine->dd, o, W);
         }, &L->rpi, &L->cii, &L->vdi, &L->ddi, &L->nnzi);
-    return galerkin(c, H, ni, [&](GalOut o) {
-        k_gal0<V><<<gal_blocks(ni), GAL_T, 0, c->stream>>>(*v0, ni, membf, aggi, o);
-    }, &L->rpi, &L->cii, &L->vdi, &L->ddi, &L->nnzi);
+    return galerkin(c, H, ni, [&](GalOut o) { launch_gal0<V>(c, *v0, ni, membf, aggi, o, W); }, &L->rpi, &L->cii,
+                    &L->vdi, &L->ddi, &L->nnzi);
 }
 
 // Galerkin matrix of L's rows from its intermediate matrix
@@ -613,8 +752,10 @@ int refill_level(hs_ctx* c, AmgHier* H, AmgLevel* L) {
     const long n = L->n;
     const int* membi = L->membi;
     const int* ab = L->aggb;
+    L->wb = std::min(MAXW, 2 * L->wbi);
+    const int W = width_class(L->wb);
     return galerkin(c, H, n, [&](GalOut o) {
-        k_gal_csr<<<gal_blocks(n), GAL_T, 0, c->stream>>>(n, membi, ab, L->rpi, L->cii, L->vdi, L->ddi, o);
+        launch_gal_csr(c, n, membi, ab, L->rpi, L->cii, L->vdi, L->ddi, o, W);
     }, &L->rp, &L->ci, &L->vd, &L->dd, &L->nnz);
 }
 
@@ -627,6 +768,10 @@ int make_level(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine, float t
     *out = nullptr;
     AmgLevel* L = new AmgLevel();
     const long nf = fine ? fine->n : H->n0;
+    // below 64K rows every coupling is a candidate: the strong sets of the small levels'
+    // large aggregates leave too many rows single for the hierarchy to reach the dense
+    // inverse's 4096 rows
+    if (nf < 65536) theta = 0.0f;
     int e = HS_OK;
     // pass A over the level above -> intermediate matrix
     if (fine) e = pair_pass_csr(c, H, nf, fine->rp, fine->ci, fine->vd, fine->dd, theta, &L->ni, &L->aggi, &L->membf);
@@ -644,7 +789,7 @@ int make_level(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine, float t
         k_compose_memb<<<blocks_for(L->n), FLAT, 0, c->stream>>>(L->n, L->membi, L->membf, L->memb);
         if (hipGetLastError() != hipSuccess) e = HS_E_HIP;
     }
-    if (e == HS_OK) e = narrow_level<T>(c, L);
+    if (e == HS_OK) e = sell_level<T>(c, H, L);
     if (e == HS_OK) e = level_vectors<T>(L);
     if (e != HS_OK || L->n >= (long)(0.9 * (double)nf)) {
         level_free(L);
@@ -725,7 +870,7 @@ int refill(hs_ctx* c, AmgHier* H, const V& v0) {
     for (AmgLevel* L : H->lv) {
         HS_TRY(refill_intermediate<V>(c, H, &v0, fine, L));
         HS_TRY(refill_level(c, H, L));
-        HS_TRY(narrow_level<T>(c, L));
+        HS_TRY(sell_level<T>(c, H, L));
         fine = L;
     }
     return HS_OK;
@@ -746,7 +891,7 @@ int cycle_csr(hs_ctx* c, AmgHier* H, size_t l, const AmgOpt& o, const int* skip)
     const T w = (T)(2.0 / 3.0);
     auto jacobi = [&](int sweeps) -> int {
         for (int s = 0; s < sweeps; ++s) {
-            k_jac<T><<<nb, FLAT, 0, c->stream>>>(L->n, L->rp, L->ci, (const T*)L->val, (const T*)L->dg,
+            k_jac<T><<<nb, FLAT, 0, c->stream>>>(L->n, L->so, L->sw, L->sc, (const T*)L->sv, (const T*)L->dg,
                                                  (const T*)L->b, (const T*)L->x, (T*)L->x2, w, skip);
             std::swap(L->x, L->x2);
         }
@@ -763,9 +908,9 @@ int cycle_csr(hs_ctx* c, AmgHier* H, size_t l, const AmgOpt& o, const int* skip)
     k_jac0<T><<<nb, FLAT, 0, c->stream>>>(L->n, (const T*)L->dg, (const T*)L->b, (T*)L->x, w, skip);
     HS_TRY(jacobi(o.sweeps - 1));
     AmgLevel* C = H->lv[l + 1];
-    k_restrict_csr<T><<<blocks_for(C->n), FLAT, 0, c->stream>>>(C->n, C->memb, L->rp, L->ci, (const T*)L->val,
-                                                              (const T*)L->dg, (const T*)L->b, (const T*)L->x,
-                                                              (T*)C->b, skip);
+    k_resid<T><<<nb, FLAT, 0, c->stream>>>(L->n, L->so, L->sw, L->sc, (const T*)L->sv, (const T*)L->dg,
+                                           (const T*)L->b, (const T*)L->x, (T*)L->r, skip);
+    k_gather4<T><<<blocks_for(C->n), FLAT, 0, c->stream>>>(C->n, C->memb, (const T*)L->r, (T*)C->b, skip);
     HS_CHECK(hipGetLastError());
     HS_TRY(cycle_csr<T>(c, H, l + 1, o, skip));
     k_prolong<T><<<nb, FLAT, 0, c->stream>>>(L->n, C->agg, (const T*)C->x, (T*)L->x, (T)o.alpha, skip);
@@ -773,11 +918,14 @@ int cycle_csr(hs_ctx* c, AmgHier* H, size_t l, const AmgOpt& o, const int* skip)
     return jacobi(o.sweeps);
 }
 
-// below the finest level: restrict its residual, cycle, add the scaled correction
+// below the finest level: its residual into r0 (the finest level's spare iterate buffer,
+// free between the smoother's sweeps), gathered onto the first aggregated level, the
+// cycle there, the scaled correction added to x0
 template <typename T, typename V>
-int coarse_part(hs_ctx* c, AmgHier* H, const V& v0, T* x0, const AmgOpt& o, const int* skip) {
+int coarse_part(hs_ctx* c, AmgHier* H, const V& v0, T* x0, T* r0, const AmgOpt& o, const int* skip) {
     AmgLevel* L1 = H->lv[0];
-    k_restrict0<V, T><<<blocks_for(L1->n), FLAT, 0, c->stream>>>(v0, L1->n, L1->memb, (T*)L1->b, skip);
+    k_resid0<V, T><<<blocks_for(H->n0), FLAT, 0, c->stream>>>(v0, H->n0, r0, skip);
+    k_gather4<T><<<blocks_for(L1->n), FLAT, 0, c->stream>>>(L1->n, L1->memb, r0, (T*)L1->b, skip);
     HS_CHECK(hipGetLastError());
     HS_TRY(cycle_csr<T>(c, H, 0, o, skip));
     k_prolong<T><<<blocks_for(H->n0), FLAT, 0, c->stream>>>(H->n0, L1->agg, (const T*)L1->x, x0, (T)o.alpha, skip);
@@ -823,10 +971,10 @@ int amg_coarse_p(hs_ctx* c, const int* skip) {
     const AmgOpt o = options(c, false);
     if (H->f32) {
         const PView<float> v{mgl::lv<float>(m)};
-        return coarse_part<float, PView<float>>(c, H, v, (float*)m->x, o, skip);
+        return coarse_part<float, PView<float>>(c, H, v, (float*)m->x, (float*)m->x2, o, skip);
     }
     const PView<double> v{mgl::lv<double>(m)};
-    return coarse_part<double, PView<double>>(c, H, v, (double*)m->x, o, skip);
+    return coarse_part<double, PView<double>>(c, H, v, (double*)m->x, (double*)m->x2, o, skip);
 }
 
 // energy: one aggregation per operator build and direction (the operator is frozen for
@@ -859,10 +1007,10 @@ int amg_coarse_e(hs_ctx* c, ELevel* l0, int tr, const int* skip) {
     const AmgOpt o = options(c, true);
     if (H->f32) {
         const EView<float> v{heo::el<float>(l0), tr};
-        return coarse_part<float, EView<float>>(c, H, v, (float*)l0->x, o, skip);
+        return coarse_part<float, EView<float>>(c, H, v, (float*)l0->x, (float*)l0->x2, o, skip);
     }
     const EView<double> v{heo::el<double>(l0), tr};
-    return coarse_part<double, EView<double>>(c, H, v, (double*)l0->x, o, skip);
+    return coarse_part<double, EView<double>>(c, H, v, (double*)l0->x, (double*)l0->x2, o, skip);
 }
 
 void amg_free(AmgHier** Hp) {

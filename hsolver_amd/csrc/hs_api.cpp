@@ -1017,6 +1017,20 @@ int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels) {
     return n;
 }
 
+int hs_amg_levels(hs_ctx* c, int32_t which, int64_t* rows_nnz, int32_t max_levels) {
+    if (!c || (which != 0 && which != 1) || max_levels < 0 || (max_levels > 0 && !rows_nnz)) return HS_E_ARG;
+    const AmgHier* H = which == 0 ? c->amgp : (c->en ? c->en->amg : nullptr);
+    if (!H) return 0;
+    int n = 0;
+    for (const AmgLevel* L : H->lv) {
+        if (n >= max_levels) break;
+        rows_nnz[2 * n] = L->n;
+        rows_nnz[2 * n + 1] = L->nnz;
+        ++n;
+    }
+    return n;
+}
+
 }  // extern "C"
 
 extern "C" {

@@ -175,8 +175,15 @@ struct AmgLevel {
     long n = 0, nnz = 0;            // rows, stored off-diagonal entries
     int *rp = nullptr, *ci = nullptr;
     double *vd = nullptr, *dd = nullptr;   // entries and diagonal in FP64 (setup, refinement)
-    void *val = nullptr, *dg = nullptr;    // the same in the cycle's scalar (alias when FP64)
-    void *x = nullptr, *x2 = nullptr, *b = nullptr;
+    void* dg = nullptr;                    // diagonal in the cycle's scalar (alias when FP64)
+    // the cycle's copy of the entries: sliced ELL, 64 rows (one wavefront) per slice, each
+    // slice padded to its longest row and stored slot-major, so that the 64 lanes of a
+    // wavefront read 64 consecutive entries
+    int *so = nullptr, *sw = nullptr, *sc = nullptr;   // slice offsets, widths, columns
+    void* sv = nullptr;                    // entries in the cycle's scalar
+    long nsell = 0;
+    int wb = 0, wbi = 0;                   // bounds of the row lengths (level, intermediate)
+    void *x = nullptr, *x2 = nullptr, *b = nullptr, *r = nullptr;
     int* agg = nullptr;             // rows of the level above -> rows of this level (-1: none)
     int* memb = nullptr;            // 4 per row: rows of the level above in this aggregate (-1: none)
     // the first pairwise pass's intermediate matrix (kept for the numeric rebuild)

@@ -2,7 +2,7 @@
 
     python -m hsolver_amd.workloads c1 [--iters 200]
     python -m hsolver_amd.workloads c2 [--iters 500]
-    python -m hsolver_amd.workloads c5 [--iters 50]
+    python -m hsolver_amd.workloads c5 [--iters 50] [--precond gmg|amg] [--energy gmg,amg]
 
 c1  64x16x16 channel, solid cube, Re 100, k ratio 100, q = 1: the reference's solver
     settings (lexicographic sweeps, unpreconditioned CG to 1e-8) and the fast path
@@ -19,9 +19,11 @@ c2  256x64x64 channel (1.05M cells), heated cube with seeded q, Re 400: the FP64
       u and a_P written on the colour, class)
 c5  512x256x256 channel (33.5M cells), y/z tanh-stretched to cell aspect ratio 50, 64
     heated plate fins (20% of the cells), Re 2000, k ratio 1000: mixed-precision SIMPLE
-    (multigrid FCG with the dense coarse solve), then on the frozen flow the energy solve
-    and the adjoint of the fins' temperature p-norm (p = 8) with respect to the fin heat
-    sources, checked by central differences on seeded fin cells.
+    (multigrid FCG with the dense coarse solve; --precond gmg|amg picks the geometric or
+    the aggregation multigrid), then on the frozen flow the energy solve and the adjoint
+    of the fins' temperature p-norm (p = 8) with respect to the fin heat sources, checked
+    by central differences on seeded fin cells, once per energy preconditioner in
+    --energy (default: geometric, then aggregation multigrid).
 Each prints one JSON line.  Inputs are synthetic (configs.py), resident in HBM.
 """
 from __future__ import annotations
@@ -114,32 +116,15 @@ def run_c2(iters):
     return out
 
 
-def run_c5(iters, fd_cells=4):
+PRECONDS = {"gmg": abi.HS_PRECOND_GMG, "amg": abi.HS_PRECOND_AMG}
+
+
+def energy_and_adjoint(ctx, case, fd_cells, tag):
+    """Energy solve, adjoint of the fins' temperature p-norm (p = 8) with respect to the fin
+    heat sources, and central differences on fd_cells seeded fin cells, on ctx's flow."""
     import numpy as np
-    from .adjoint import central_difference, cell_volumes, pnorm_difference  # noqa: F401
-    case = configs.c5()
+    from .adjoint import central_difference, cell_volumes
     nx, ny, nz = case.shape
-    out = {"workload": case.workload, "cells": nx * ny * nz, "solid_fraction": round(float(case.grid.cell_type.mean()), 4),
-           "max_cell_aspect": round(configs.max_cell_aspect(case.grid.xv, case.grid.yv, case.grid.zv), 2),
-           "iters": iters}
-    p = abi.Params.from_buffer_copy(case.params)
-    p.order, p.precond, p.precision = abi.HS_ORDER_RB, abi.HS_PRECOND_GMG, abi.HS_PREC_MIXED
-    p.mg_dense_coarse = 50
-    ctx = core.create(case.grid, p, case.q)
-    ctx.init()
-    for w in range(2):
-        st = ctx.step()
-        log(f"c5 warm-up {w + 1}: p' iters {st['p_iters']}, true relres {st['p_true_relres']:.2e}")
-    hist = []
-    t0 = time.perf_counter()
-    for s in range(iters):
-        hist.append(ctx.step())
-        if (s + 1) % 10 == 0:
-            log(f"c5: {s + 1}/{iters} SIMPLE iterations, p' iters {hist[-1]['p_iters']}, "
-                f"mass imbalance {hist[-1]['mass_imbalance']:.3e}")
-    dt = time.perf_counter() - t0
-    out["mixed_simple"] = summary(dt, hist, iters)
-    # adjoint of the fins' temperature p-norm on the frozen flow
     shape = (nz, ny, nx)
     solid = case.grid.cell_type.reshape(shape) == 1
     V = cell_volumes(case)
@@ -153,7 +138,7 @@ def run_c5(iters, fd_cells=4):
     t0 = time.perf_counter()
     dq, dk, adj = ctx.adjoint_energy(dJdT, 1e-11, 2000)
     t_adj = time.perf_counter() - t0
-    log(f"c5 energy solve {fwd}, {t_fwd:.2f} s; adjoint {adj}, {t_adj:.2f} s; J = {J:.6e}")
+    log(f"c5 [{tag}] energy solve {fwd}, {t_fwd:.2f} s; adjoint {adj}, {t_adj:.2f} s; J = {J:.6e}")
     q0, k0 = ctx.cells("q").copy(), ctx.cells("k").copy()
     idx = np.flatnonzero(solid.ravel())
     pick = idx[(configs.splitmix64(55, fd_cells) * idx.size).astype(np.int64)]
@@ -162,10 +147,55 @@ def run_c5(iters, fd_cells=4):
         cell = np.unravel_index(c, shape)
         fd = central_difference(ctx, "q", cell, 1e-6 * q0[cell], q0, k0, solid, V, tol=1e-11, maxit=2000)
         errs.append(abs(fd - dq[cell]) / abs(dq[cell]))
-        log(f"c5 central difference at fin cell {tuple(int(v) for v in cell)}: {fd:.9e} vs adjoint {dq[cell]:.9e}")
+        log(f"c5 [{tag}] central difference at fin cell {tuple(int(v) for v in cell)}: {fd:.9e} vs adjoint {dq[cell]:.9e}")
     ctx.put_cells("q", q0)
-    out["adjoint"] = {"J": J, "forward": fwd, "adjoint": adj, "forward_s": round(t_fwd, 3),
-                      "adjoint_s": round(t_adj, 3), "fd_cells": fd_cells, "fd_max_rel_err_q": max(errs)}
+    return {"J": J, "forward": fwd, "adjoint": adj, "forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3),
+            "fd_cells": fd_cells, "fd_max_rel_err_q": max(errs) if errs else None}
+
+
+def run_c5(iters, fd_cells=4, precond="gmg", energy=("gmg", "amg")):
+    case = configs.c5()
+    nx, ny, nz = case.shape
+    out = {"workload": case.workload, "cells": nx * ny * nz, "solid_fraction": round(float(case.grid.cell_type.mean()), 4),
+           "max_cell_aspect": round(configs.max_cell_aspect(case.grid.xv, case.grid.yv, case.grid.zv), 2),
+           "iters": iters, "p_precond": precond}
+    p = abi.Params.from_buffer_copy(case.params)
+    p.order, p.precond, p.precision = abi.HS_ORDER_RB, PRECONDS[precond], abi.HS_PREC_MIXED
+    p.mg_dense_coarse = 50
+    p.e_precond = PRECONDS[energy[0]]
+    ctx = core.create(case.grid, p, case.q)
+    ctx.init()
+    for w in range(2):
+        t0 = time.perf_counter()
+        st = ctx.step()
+        log(f"c5 warm-up {w + 1}: p' iters {st['p_iters']}, true relres {st['p_true_relres']:.2e}, "
+            f"{time.perf_counter() - t0:.2f} s")
+    hist = []
+    t0 = time.perf_counter()
+    for s in range(iters):
+        hist.append(ctx.step())
+        if (s + 1) % 10 == 0:
+            log(f"c5: {s + 1}/{iters} SIMPLE iterations, p' iters {hist[-1]['p_iters']}, "
+                f"mass imbalance {hist[-1]['mass_imbalance']:.3e}")
+    dt = time.perf_counter() - t0
+    out["mixed_simple"] = summary(dt, hist, iters)
+    if precond == "amg":
+        out["mixed_simple"]["amg_levels"] = ctx.amg_levels(0)
+    # energy solve and adjoint on the frozen flow, with each energy preconditioner asked for
+    for k, name in enumerate(energy):
+        if k == 0:
+            c2 = ctx
+        else:
+            p2 = abi.Params.from_buffer_copy(p)
+            p2.e_precond = PRECONDS[name]
+            c2 = core.create(case.grid, p2, case.q)
+            c2.init()
+            c2.load_state(ctx.state())
+        out[f"adjoint_{name}"] = energy_and_adjoint(c2, case, fd_cells, name)
+        if name == "amg":
+            out[f"adjoint_{name}"]["amg_levels"] = c2.amg_levels(1)
+        if k > 0:
+            c2.close()
     ctx.close()
     return out
 
@@ -174,13 +204,17 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("config", choices=["c1", "c2", "c5"])
     ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--precond", choices=sorted(PRECONDS), default="gmg", help="c5: preconditioner of the p' solves")
+    ap.add_argument("--energy", default="gmg,amg",
+                    help="c5: energy / adjoint preconditioners to run on the frozen flow, comma-separated")
+    ap.add_argument("--fd-cells", type=int, default=4, help="c5: central differences per energy preconditioner")
     args = ap.parse_args(argv)
     if args.config == "c1":
         out = run_c1(args.iters or 200)
     elif args.config == "c2":
         out = run_c2(args.iters or 500)
     else:
-        out = run_c5(args.iters or 50)
+        out = run_c5(args.iters or 50, args.fd_cells, args.precond, tuple(args.energy.split(",")))
     print(json.dumps(out), flush=True)
     return 0
 

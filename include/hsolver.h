@@ -348,6 +348,10 @@ int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms);
 int hs_copy_bench(int64_t nbytes, int32_t reps, double* avg_ms);
 /* multigrid level dimensions (3 ints per level); returns the number of levels */
 int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels);
+/* aggregation multigrid levels below the finest one (which = 0: the p' hierarchy, 1: the
+ * energy hierarchy): rows and stored off-diagonal entries, 2 per level, coarsest last;
+ * returns the number of levels (0 before the first build) */
+int hs_amg_levels(hs_ctx* c, int32_t which, int64_t* rows_nnz, int32_t max_levels);
 
 #ifdef __cplusplus
 }
