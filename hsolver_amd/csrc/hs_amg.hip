@@ -367,10 +367,16 @@ __device__ __forceinline__ void gal_store(const GalOut& o, long I, int (*cols)[G
         if (over) o.flag[0] = 1;
         return;
     }
-    const int at = o.rp[I];
-    for (int t = 0; t < m; ++t) {
+    // the refills of kept aggregates meet the same columns as the count pass (the pattern
+    // follows the aggregation-time map, not the values); never write past the row's slots
+    const int at = o.rp[I], cap = o.rp[I + 1] - at;
+    for (int t = 0; t < m && t < cap; ++t) {
         o.ci[at + t] = cols[t][t0];
         o.vd[at + t] = vals[t][t0];
+    }
+    for (int t = m; t < cap; ++t) {   // (only after a change of the pattern) zero entries on the diagonal
+        o.ci[at + t] = (int)I;
+        o.vd[at + t] = 0.0;
     }
     o.dd[I] = diag;
 }
@@ -391,9 +397,11 @@ __global__ __launch_bounds__(GAL_T) void k_gal0(V v, long nc, const int* memb2, 
         double a[6];
         v.coup(r, a);
         for (int d = 0; d < 6; ++d) {
-            const long nb = r + v.off(d);
-            if (!v.active(nb)) continue;
-            const int J = agg[nb];
+            // a neighbour is an unknown of the hierarchy iff the aggregation gave it an
+            // aggregate: the finest level's own test (D > 0 in the level's scalar) depends on
+            // the values, which change between the refills of kept aggregates
+            const int J = agg[r + v.off(d)];
+            if (J < 0) continue;
             if (J == (int)I) diag -= a[d];
             else gal_add<W>(cols, vals, m, over, J, -a[d]);
         }
@@ -815,9 +823,9 @@ struct AmgOpt {
 AmgOpt options(const hs_ctx* c, bool energy) {
     const hs_params& p = c->prm;
     AmgOpt o;
-    o.theta = p.amg_theta > 0.0f ? p.amg_theta : 0.25f;
-    o.alpha = p.amg_alpha > 0.0f ? (double)p.amg_alpha : (energy ? 1.6 : 1.0);
-    o.sweeps = p.amg_sweeps > 0 ? p.amg_sweeps : 2;
+    o.theta = p.amg_theta;     // likewise
+    o.alpha = p.amg_alpha > 0.0f ? (double)p.amg_alpha : (energy ? 1.6 : 1.4);
+    o.sweeps = p.amg_sweeps;   // chosen at hs_create when left <= 0
     o.refine = std::max(1, p.mg_dense_refine);
     o.coarse = std::max(1, p.mg_coarse_sweeps);
     return o;

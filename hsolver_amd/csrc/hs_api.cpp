@@ -406,12 +406,17 @@ static void resolve_sweeps(hs_ctx* c) {
     // 94 + 105 BiCGStab iterations on one frozen flow and left the adjoint diverging on a
     // later one
     if (!(p.e_omega > 0.0f)) p.e_omega = 1.0f;
-    // aggregation multigrid: one red-black sweep before and after on the finest level, two
-    // Jacobi sweeps on every aggregated level, theta 0.25, Gauss-Seidel on the finest level
-    // (the coarse-correction scaling stays <= 0: 1 for p', 1.6 for the energy solves)
-    if (p.amg_sweeps0 <= 0) p.amg_sweeps0 = 1;
-    if (p.amg_sweeps <= 0) p.amg_sweeps = 2;
-    if (!(p.amg_theta > 0.0f)) p.amg_theta = 0.25f;
+    // aggregation multigrid, measured on the 33.5M-cell finned channel (aspect 50, k ratio
+    // 1000): two red-black sweeps before and after on the finest level, three Jacobi sweeps
+    // on every aggregated level, theta 0.1, the prolongated correction scaled by 1.4 (p') or
+    // 1.6 (energy; left <= 0 here and chosen by use).  Energy solve: (1, 2) sweeps at scaling
+    // 1 took 242 BiCGStab iterations, at 1.6 109; (2, 3) at 1.6 68, with theta 0.1 59
+    // (1.16 s against the geometric hierarchy's 163 in 2.63 s).  p' solve (FCG per SIMPLE
+    // iteration): (1, 2) at 1: 48 iterations, (2, 3) at 1: 42, at 1.4: 26, at 1.6: 29, at
+    // 1.8: 38
+    if (p.amg_sweeps0 <= 0) p.amg_sweeps0 = 2;
+    if (p.amg_sweeps <= 0) p.amg_sweeps = 3;
+    if (!(p.amg_theta > 0.0f)) p.amg_theta = 0.1f;
     if (!(p.amg_omega > 0.0f)) p.amg_omega = 1.0f;
     if (p.e_precond == 0) p.e_precond = HS_PRECOND_GMG;
 }

@@ -397,7 +397,7 @@ __global__ void k_dense_assemble_csr(int n, const int* rp, const int* ci, const 
     T* row = M + (long)r * np;
     if (r >= n) { row[r] = T(1); return; }
     row[r] = (T)dd[r];
-    for (int t = rp[r]; t < rp[r + 1]; ++t) row[ci[t]] = (T)vd[t];
+    for (int t = rp[r]; t < rp[r + 1]; ++t) row[ci[t]] += (T)vd[t];
 }
 
 // r = b (FP64), xf = b in the inverse's scalar; padding rows 0

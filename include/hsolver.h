@@ -152,15 +152,17 @@ typedef struct {
        strong couplings, Galerkin coarse operators, a dense inverse on the matrix cores once
        a level has at most 4096 rows (FP64 refinement), V-cycle */
     int32_t amg_sweeps0;              /* red-black Gauss-Seidel sweeps on the finest level before and
-                                         after the coarse correction (<= 0: 1) */
+                                         after the coarse correction (<= 0: 2) */
     int32_t amg_sweeps;               /* Jacobi sweeps (weight 2/3) before and after on every
-                                         aggregated level (<= 0: 2) */
+                                         aggregated level (<= 0: 3) */
     float   amg_theta;                /* strong coupling: the pair's symmetrised coupling over the
                                          geometric mean of the two diagonals is at least this
-                                         fraction of the row's largest (<= 0: 0.25) */
-    float   amg_alpha;                /* scaling of every prolongated coarse correction (<= 0: 1 for
-                                         the p' solve, 1.6 for the energy solves) */
-    float   amg_omega;                /* over-relaxation of the finest level's sweeps (<= 0: 1) */
+                                         fraction of the row's largest (<= 0: 0.1; every coupling
+                                         below 64K rows) */
+    float   amg_alpha;                /* scaling of every prolongated coarse correction (<= 0: 1.4
+                                         for the p' solve, 1.6 for the energy solves) */
+    float   amg_omega;                /* over-relaxation of the p' cycle's finest-level sweeps (<= 0:
+                                         1; the energy cycle's use e_omega) */
     int32_t amg_pad;
 } hs_params;
 

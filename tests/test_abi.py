@@ -106,6 +106,7 @@ def test_null_context_is_an_argument_error(hs_lib):
         "hs_energy_solve_rhs": (0, None, None, C.c_double(1e-8), 10, 0, None),
         "hs_energy_apply": (0, None, None, 0), "hs_get_cells": (0, None, 0), "hs_put_cells": (0, None, 0),
         "hs_prof_set": (0,), "hs_prof_get": (None, None), "hs_mg_levels": (None, 0), "hs_get_params": (None,),
+        "hs_amg_levels": (0, None, 0),
     }
     for name, rest in calls.items():
         fn = getattr(hs_lib, name)
@@ -134,3 +135,16 @@ def test_wrappers_check_buffer_sizes(oracle_lib):
             call()
     ctx.set_source(np.ones(nc))   # the right size passes
     ctx.close()
+
+
+def test_create_refuses_unknown_preconditioners(hs_lib):
+    """precond and e_precond outside none / geometric / aggregation multigrid are refused
+    before any device call."""
+    case = configs.c2(scale=16)
+    for field in ("precond", "e_precond"):
+        for bad in (-1, 3):
+            p = abi.Params.from_buffer_copy(case.params)
+            setattr(p, field, bad)
+            h = C.c_void_p()
+            assert hs_lib.hs_create(C.byref(case.grid.struct), C.byref(p), C.byref(h)) == abi.HS_E_ARG, (field, bad)
+            assert not h.value

@@ -163,6 +163,10 @@ def test_smoothing_chosen_by_grid(hs_lib):
     assert (r.mg_pre0, r.mg_post0, r.mg_pre, r.mg_post) == (1, 2, 3, 3)
     assert (round(r.mg_omega, 4), round(r.mg_omega_coarse, 4), r.e_omega) == (1.2, 1.6, 1.0)
     assert (r.e_pre, r.e_post) == (3, 3)
+    # aggregation multigrid: two finest-level sweeps, three Jacobi sweeps, theta 0.1,
+    # Gauss-Seidel on the finest level; the correction scaling stays chosen by use
+    assert (r.amg_sweeps0, r.amg_sweeps, round(r.amg_theta, 4), r.amg_omega, r.amg_alpha) == (2, 3, 0.1, 1.0, 0.0)
+    assert r.e_precond == abi.HS_PRECOND_GMG
     st = configs.c5(32, 16, 16, nfins=2)
     assert configs.max_cell_aspect(st.grid.xv, st.grid.yv, st.grid.zv) > 8
     p = abi.Params.from_buffer_copy(st.params)
