@@ -51,7 +51,7 @@ def log(msg: str) -> None:
 
 
 def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=None, dense=0, fields_out=None,
-            step_ms=None, warm=0):
+            step_ms=None, warm=0, energy_out=None):
     from hsolver_amd import abi, core
     lib = core.library()
     lib.hs_prof_set.argtypes = [C.c_void_p, C.c_int32]
@@ -62,6 +62,7 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
     p.order = abi.HS_ORDER_RB if order is None else order
     p.mg_dense_coarse = dense
     p.p_warm = warm
+    p.e_precond = abi.HS_PRECOND_AMG         # only the energy solve after the timed region uses it
     ctx = core.create(case.grid, p, case.q)
     ctx.init()
     for w in range(warmup):
@@ -87,6 +88,17 @@ def gpu_run(case, precision, precond, steps, warmup, prof_kernel=None, order=Non
         lib.hs_prof_set(ctx.h, 0)
     if fields_out is not None:
         fields_out.update(ctx.fields())      # after the timed region
+    if energy_out is not None:
+        # after the timed region: the global energy balance the SIMPLE sweeps left, then the
+        # conjugate energy equation solved on the final (frozen) flow and its balance
+        _, r0 = ctx.energy_balance()
+        te = time.perf_counter()
+        info = ctx.energy_solve(1e-10, 1000)
+        te = time.perf_counter() - te
+        _, r1 = ctx.energy_balance()
+        energy_out.update({"after_simple_steps": r0, "after_energy_solve": r1,
+                           "energy_solve": {"preconditioner": "aggregation multigrid", "iters": info["iters"],
+                                            "true_relres": info["true_relres"], "s": round(te, 3)}})
     ctx.close()
     return (t1 - t0), stats, prof
 
@@ -198,9 +210,12 @@ def main() -> int:
 
     f_mixed, f_fp64 = {}, {}
     ms_mixed, ms_fp64 = [], []
+    energy = {}
     dt_mixed, st_mixed, prof = gpu_run(case, abi.HS_PREC_MIXED, abi.HS_PRECOND_GMG, args.steps, args.warmup,
                                        prof_kernel=1, dense=args.dense, fields_out=f_mixed, step_ms=ms_mixed,
-                                       warm=0 if args.cold else 3)
+                                       warm=0 if args.cold else 3, energy_out=energy)
+    log(f"energy balance after the SIMPLE steps {energy['after_simple_steps']:.3e}, after the energy solve "
+        f"{energy['after_energy_solve']:.3e} ({energy['energy_solve']})")
     value = args.steps / dt_mixed
     log(f"mixed: {value:.2f} SIMPLE it/s, p' iters {[s['p_iters'] for s in st_mixed]}")
 
@@ -271,6 +286,7 @@ def main() -> int:
         "p_true_relres_max": max(relres) if relres else None,
         "mass_imbalance_last": st_mixed[-1]["mass_imbalance"] if st_mixed else None,
         "energy_residual_last": st_mixed[-1]["energy_residual"] if st_mixed else None,
+        "energy_balance": energy,
         "fp64_cg_first_step": cg_step,
         "roofline": roofline,
         "cpu_baseline": cpu,

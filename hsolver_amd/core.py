@@ -238,6 +238,14 @@ class Context:
         self._call("energy_apply", 1 if transpose else 0, xa.ctypes.data, y.ctypes.data, abi.HS_MEM_HOST)
         return y.reshape(self.nz, self.ny, self.nx)
 
+    def energy_balance(self) -> tuple:
+        """(heat input sum q V, |sum of energy residuals| / heat input) of the current field."""
+        fn = self.lib.hs_energy_balance
+        fn.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        h, r = C.c_double(), C.c_double()
+        self._call("energy_balance", C.byref(h), C.byref(r))
+        return h.value, r.value
+
     def amg_levels(self, which: int = 0) -> list:
         """(rows, off-diagonal entries) of each aggregated level: which = 0 the p' hierarchy,
         1 the energy hierarchy."""

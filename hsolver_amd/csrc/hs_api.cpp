@@ -1022,6 +1022,17 @@ int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels) {
     return n;
 }
 
+int hs_energy_balance(hs_ctx* c, double* heat_in, double* residual) {
+    if (!c || !heat_in || !residual) return HS_E_ARG;
+    if (c->prm.mode != HS_MODE_CONFIG) return HS_E_ARG;
+    if (!c->inited) return HS_E_STATE;
+    double heat = 0.0, res = 0.0;
+    HS_TRY(energy_balance(c, &heat, &res));
+    *heat_in = heat;
+    *residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
+    return HS_OK;
+}
+
 int hs_amg_levels(hs_ctx* c, int32_t which, int64_t* rows_nnz, int32_t max_levels) {
     if (!c || (which != 0 && which != 1) || max_levels < 0 || (max_levels > 0 && !rows_nnz)) return HS_E_ARG;
     const AmgHier* H = which == 0 ? c->amgp : (c->en ? c->en->amg : nullptr);

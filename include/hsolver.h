@@ -330,6 +330,10 @@ int hs_energy_solve_rhs(hs_ctx* c, int32_t transpose, const double* b, double* x
                         int32_t maxit, int32_t mem, hs_solve_info* info);
 /* y = A x (transpose = 0) or A^T x (transpose = 1) of the energy operator. */
 int hs_energy_apply(hs_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem);
+/* Global energy balance of the current temperature field (Mode C): *heat_in = sum q V,
+ * *residual = |sum over cells of the energy equation's residual| / |heat_in|, the figure
+ * hs_stats.energy_residual reports after each hs_step. */
+int hs_energy_balance(hs_ctx* c, double* heat_in, double* residual);
 /* Cell arrays in the C layout: HS_A_TF (temperature, interior), HS_A_Q, HS_A_K. */
 int hs_get_cells(hs_ctx* c, int32_t which, double* dst, int32_t mem);
 int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem);

@@ -106,7 +106,7 @@ def test_null_context_is_an_argument_error(hs_lib):
         "hs_energy_solve_rhs": (0, None, None, C.c_double(1e-8), 10, 0, None),
         "hs_energy_apply": (0, None, None, 0), "hs_get_cells": (0, None, 0), "hs_put_cells": (0, None, 0),
         "hs_prof_set": (0,), "hs_prof_get": (None, None), "hs_mg_levels": (None, 0), "hs_get_params": (None,),
-        "hs_amg_levels": (0, None, 0),
+        "hs_amg_levels": (0, None, 0), "hs_energy_balance": (None, None),
     }
     for name, rest in calls.items():
         fn = getattr(hs_lib, name)

@@ -15,7 +15,7 @@ import pytest
 
 from hsolver_amd import abi, configs
 
-from helpers import cell_volumes, gpu_ctx, oracle_ctx, rel_l2, solid_pnorm
+from helpers import cell_volumes, energy_balance, gpu_ctx, oracle_ctx, rel_l2, solid_pnorm
 
 pytestmark = pytest.mark.gpu
 
@@ -171,4 +171,25 @@ def test_amg_adjoint_matches_oracle(hs_lib, frozen, precision):
     assert info["converged"] == 1 and info_o["converged"] == 1
     assert rel_l2(dq, dqo) < 1e-9
     assert rel_l2(dk, dko) < 1e-8
+    g.close()
+
+
+def test_energy_balance_after_amg_solve(hs_lib, frozen):
+    """hs_energy_balance on the frozen flow's temperature: the global balance that the SIMPLE
+    sweeps leave open closes after the energy solve, and the heat input and outflow agree
+    with the physics sum built in numpy (heat in = sum q V; out = outflow enthalpy and
+    Dirichlet-face conduction, with the residual mass imbalance's share)."""
+    case, st, o = frozen
+    g = energy_ctx(hs_lib, case, st, abi.HS_PREC_MIXED)
+    h0, r0 = g.energy_balance()
+    info = g.energy_solve(tol=1e-12, maxit=400)
+    h1, r1 = g.energy_balance()
+    q_in, q_out, q_mass = energy_balance(case, g.cells("tf"), g.get("u"), g.get("v"), g.get("w"), g.cells("k"),
+                                         g.cells("q"))
+    print(f"[amg] energy balance before {r0:.3e}, after the solve {r1:.3e} ({info['iters']} iterations); "
+          f"numpy: in {q_in:.12e} out {q_out:.12e} mass share {q_mass:.3e}", flush=True)
+    assert info["converged"] == 1
+    assert h0 == h1 and abs(h1 - q_in) < 1e-12 * q_in
+    assert r0 > 1e-3 and r1 < 1e-9
+    assert abs(q_in - q_out + q_mass) < 1e-8 * q_in
     g.close()
