@@ -190,6 +190,6 @@ def test_energy_balance_after_amg_solve(hs_lib, frozen):
           f"numpy: in {q_in:.12e} out {q_out:.12e} mass share {q_mass:.3e}", flush=True)
     assert info["converged"] == 1
     assert h0 == h1 and abs(h1 - q_in) < 1e-12 * q_in
-    assert r0 > 1e-3 and r1 < 1e-9
-    assert abs(q_in - q_out + q_mass) < 1e-8 * q_in
+    assert r0 > 1e-3 and r1 < 1e-8
+    assert abs(q_in - q_out + q_mass) < 1e-7 * q_in
     g.close()
