@@ -587,6 +587,10 @@ int ensure_scratch(AmgHier* H, long n) {
 
 // exclusive scan of in[n] into out[n]; returns the total (synchronises)
 int scan(hs_ctx* c, AmgHier* H, long n, const int* in, int* out, int* bsum, long* total) {
+    if (n <= 0) {
+        *total = 0;
+        return HS_OK;
+    }
     const int nb = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
     k_scan_local<<<nb, SCAN_T, 0, c->stream>>>(n, in, out, bsum);
     k_scan_top<<<1, 1024, 0, c->stream>>>(nb, bsum, H->dflag + 1);
@@ -673,7 +677,7 @@ int galerkin(hs_ctx* c, AmgHier* H, long nc, F launch, int** rp, int** ci, doubl
     return HS_OK;
 }
 
-inline int gal_blocks(long nc) { return (int)((nc + GAL_T - 1) / GAL_T); }
+inline int gal_blocks(long nc) { return (int)std::max(1L, (nc + GAL_T - 1) / GAL_T); }
 
 // the cycle's copies of a level's entries: the sliced ELL form (pattern built on the first
 // call, values refilled on later ones) and the diagonal (an alias in an FP64 hierarchy)
@@ -784,6 +788,10 @@ int make_level(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine, float t
     // pass A over the level above -> intermediate matrix
     if (fine) e = pair_pass_csr(c, H, nf, fine->rp, fine->ci, fine->vd, fine->dd, theta, &L->ni, &L->aggi, &L->membf);
     else e = pair_pass0<V>(c, H, *v0, nf, theta, &L->ni, &L->aggi, &L->membf);
+    if (e == HS_OK && L->ni == 0) {   // no unknowns on the level above: nothing to coarsen
+        level_free(L);
+        return HS_OK;
+    }
     if (e == HS_OK) e = refill_intermediate<V>(c, H, v0, fine, L);
     // pass B over the intermediate matrix -> the new level
     if (e == HS_OK) e = pair_pass_csr(c, H, L->ni, L->rpi, L->cii, L->vdi, L->ddi, theta, &L->n, &L->aggb, &L->membi);
@@ -842,6 +850,8 @@ int hier_alloc(AmgHier** Hp, bool f32, int kind, long n0) {
     if (H->f32 != f32 || H->n0 != n0 || H->kind != kind) {
         levels_free(H);
         dense_release(&H->dense);
+        H->aggregations = 0;
+        H->built_id = -1;
     }
     H->f32 = f32;
     H->kind = kind;
@@ -864,7 +874,8 @@ int aggregate(hs_ctx* c, AmgHier* H, const V& v0, const AmgOpt& o) {
         fine = L;
         if (L->n <= dense_max_cells()) break;
     }
-    if (H->lv.empty()) return HS_E_ARG;   // nothing to coarsen (every row single and isolated)
+    // an empty hierarchy (no unknowns, or no level coarsens) leaves the cycle to the finest
+    // level's sweeps
     H->aggregations += 1;
     H->since = 0;
     H->iters_ref = -1;
@@ -885,6 +896,7 @@ int refill(hs_ctx* c, AmgHier* H, const V& v0) {
 }
 
 int factor_coarsest(hs_ctx* c, AmgHier* H) {
+    if (H->lv.empty()) return HS_OK;
     AmgLevel* L = H->lv.back();
     if (L->n > dense_max_cells()) return HS_OK;
     return dense_factor_csr(c, &H->dense, (int)L->n, L->rp, L->ci, L->vd, L->dd, H->f32);
@@ -954,7 +966,7 @@ int setup_p(hs_ctx* c) {
     const AmgOpt o = options(c, false);
     const int every = c->prm.mg_dense_coarse > 0 ? c->prm.mg_dense_coarse : 1;
     const bool stale = H->iters_ref >= 0 && c->last_iters >= H->iters_ref + 2;
-    if (H->lv.empty() || H->since >= every || stale) {
+    if (H->aggregations == 0 || H->since >= every || stale) {
         HS_TRY((aggregate<T, PView<T>>(c, H, v, o)));
         HS_TRY(factor_coarsest(c, H));
     } else {
@@ -974,7 +986,8 @@ int amg_setup_p(hs_ctx* c) {
 
 int amg_coarse_p(hs_ctx* c, const int* skip) {
     AmgHier* H = c->amgp;
-    if (!H || H->lv.empty() || c->mg.empty()) return HS_E_STATE;
+    if (!H || c->mg.empty()) return HS_E_STATE;
+    if (H->lv.empty()) return HS_OK;
     const MGLevel* m = c->mg[0];
     const AmgOpt o = options(c, false);
     if (H->f32) {
@@ -993,7 +1006,7 @@ int amg_setup_e(hs_ctx* c, ELevel* l0, int tr) {
     const int kind = tr ? 2 : 1;
     HS_TRY(hier_alloc(&E->amg, E->f32, kind, l0->n));
     AmgHier* H = E->amg;
-    if (!H->lv.empty() && H->built_id == E->build_id) return HS_OK;
+    if (H->built_id == E->build_id) return HS_OK;
     const AmgOpt o = options(c, true);
     if (E->f32) {
         const EView<float> v{heo::el<float>(l0), tr};
@@ -1011,7 +1024,8 @@ int amg_setup_e(hs_ctx* c, ELevel* l0, int tr) {
 int amg_coarse_e(hs_ctx* c, ELevel* l0, int tr, const int* skip) {
     EnergyState* E = c->en;
     AmgHier* H = E ? E->amg : nullptr;
-    if (!H || H->lv.empty() || H->kind != (tr ? 2 : 1)) return HS_E_STATE;
+    if (!H || H->kind != (tr ? 2 : 1)) return HS_E_STATE;
+    if (H->lv.empty()) return HS_OK;
     const AmgOpt o = options(c, true);
     if (H->f32) {
         const EView<float> v{heo::el<float>(l0), tr};

@@ -6,8 +6,8 @@
     (hs_create refuses fewer than three cells along any axis, tests/test_abi.py);
   * a ragged 33x17x9 box with a block touching two walls.
 
-For each, a few SIMPLE iterations of the multigrid path (FP64 and mixed) and of the drop-in
-CG against the oracle's red-black sweeps with its CG: fields within 1e-6 relative L2 (the
+For each, a few SIMPLE iterations of the geometric and the aggregation multigrid paths (FP64
+and mixed) and of the drop-in CG against the oracle's red-black sweeps with its CG: fields within 1e-6 relative L2 (the
 two Krylov solvers meet the same stop rule), the empty system finishing without iterations.
 """
 import numpy as np
@@ -42,6 +42,8 @@ PATHS = {
     "gmg_fp64": (abi.HS_PRECOND_GMG, abi.HS_PREC_FP64),
     "gmg_mixed": (abi.HS_PRECOND_GMG, abi.HS_PREC_MIXED),
     "dropin_cg": (abi.HS_PRECOND_NONE, abi.HS_PREC_FP64),
+    "amg_fp64": (abi.HS_PRECOND_AMG, abi.HS_PREC_FP64),
+    "amg_mixed": (abi.HS_PRECOND_AMG, abi.HS_PREC_MIXED),
 }
 
 
@@ -53,7 +55,7 @@ def test_degenerate_and_ragged_grids(hs_lib, oracle_lib, name, path):
     case.params.p_tol = 1e-10
     gp = abi.Params.from_buffer_copy(case.params)
     gp.precond, gp.precision = PATHS[path]
-    gp.mg_dense_coarse = 10 if gp.precision == abi.HS_PREC_MIXED else 0
+    gp.mg_dense_coarse = 10 if gp.precision == abi.HS_PREC_MIXED or gp.precond == abi.HS_PRECOND_AMG else 0
     print(f"[edge] {name} {case.shape} {path}", flush=True)
     g, o = gpu_ctx(hs_lib, case, gp), oracle_ctx(oracle_lib, case)
     g.init()
