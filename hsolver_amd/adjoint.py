@@ -221,6 +221,8 @@ def main(argv=None) -> int:
     ap.add_argument("--mass-tol", type=float, default=1e-9)
     ap.add_argument("--cells", type=int, default=32, help="seeded solid cells for central differences")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--energy-precond", choices=["gmg", "amg"], default="gmg",
+                    help="preconditioner of the energy and adjoint BiCGStab: geometric or aggregation multigrid")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -231,6 +233,7 @@ def main(argv=None) -> int:
     p.order = abi.HS_ORDER_RB
     p.precond = abi.HS_PRECOND_GMG
     p.precision = abi.HS_PREC_MIXED if args.precision == "mixed" else abi.HS_PREC_FP64
+    p.e_precond = abi.HS_PRECOND_AMG if args.energy_precond == "amg" else abi.HS_PRECOND_GMG
     ctx = core.create(case.grid, p, case.q)
     ctx.init()
     t0 = time.perf_counter()
@@ -282,7 +285,7 @@ def main(argv=None) -> int:
            "J": J.item(), "forward": fwd_info, "adjoint": adj_info,
            "forward_ms": round(t_fwd * 1e3, 2), "adjoint_ms": round(t_adj * 1e3, 2),
            "fd_cells": int(args.cells), "fd_max_rel_err_q": max(errs_q), "fd_max_rel_err_k": max(errs_k),
-           "precision": args.precision}
+           "precision": args.precision, "energy_precond": args.energy_precond}
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

@@ -3,7 +3,7 @@ callers that cannot allocate) against hs_pressure_solve on the same context and
 right-hand side: the same kernels on the same data, so the solutions and statistics
 must be identical.  Two calls in a row per solver, the second on a smaller grid; a
 multigrid context whose hierarchy does not exist yet and a grid whose reductions need
-more partials than the array holds are refused."""
+more partials than the array holds are refused, and so is the aggregation multigrid."""
 import ctypes as C
 
 import numpy as np
@@ -69,6 +69,18 @@ def test_static_refuses_missing_hierarchy(hs_lib):
     g = channel(hs_lib, 8, abi.HS_PRECOND_GMG, abi.HS_PREC_MIXED, steps=0)
     rc, _, _ = static_solve(hs_lib, g, np.zeros(abi.padded_shape("b", g.nx, g.ny, g.nz)))
     assert rc == abi.HS_E_STATE
+    g.close()
+
+
+def test_static_refuses_aggregation_multigrid(hs_lib):
+    """The aggregation multigrid allocates when it aggregates again, so the no-allocation
+    entry point refuses it (the context's own solve still works)."""
+    g = channel(hs_lib, 8, abi.HS_PRECOND_AMG, abi.HS_PREC_MIXED)
+    b = np.random.default_rng(6).standard_normal(abi.padded_shape("b", g.nx, g.ny, g.nz))
+    rc, _, _ = static_solve(hs_lib, g, b)
+    assert rc == abi.HS_E_ARG
+    _, st = g.pressure_solve(b)
+    assert st["p_converged"] == 1
     g.close()
 
 
