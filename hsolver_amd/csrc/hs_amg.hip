@@ -186,19 +186,27 @@ __device__ __forceinline__ double csr_strength(int i, int t, const int* rp, cons
     return (sym > 0.0 && di > 0.0 && dj > 0.0) ? sym / sqrt(di * dj) : 0.0;
 }
 
-__global__ __launch_bounds__(FLAT) void k_propose_csr(long n, const int* rp, const int* ci, const double* vd,
-                                                      const double* dd, const int* mate, int* prop, float theta,
-                                                      int round) {
+// strength of every stored entry, once per pass (the rounds then read it); entries of one
+// row sit at rp[i] .. rp[i+1] in the order of the matrix
+__global__ __launch_bounds__(FLAT) void k_strength_csr(long n, const int* rp, const int* ci, const double* vd,
+                                                       const double* dd, double* st) {
+    G_STRIDE(i, n) {
+        for (int t = rp[i]; t < rp[i + 1]; ++t) st[t] = csr_strength((int)i, t, rp, ci, vd, dd);
+    }
+}
+
+__global__ __launch_bounds__(FLAT) void k_propose_csr(long n, const int* rp, const int* ci, const double* st,
+                                                      const int* mate, int* prop, float theta, int round) {
     G_STRIDE(i, n) {
         int best = -1;
         if (mate[i] < 0 && is_blue(i, round)) {
             double smax = 0.0;
-            for (int t = rp[i]; t < rp[i + 1]; ++t) smax = fmax(smax, csr_strength((int)i, t, rp, ci, vd, dd));
+            for (int t = rp[i]; t < rp[i + 1]; ++t) smax = fmax(smax, st[t]);
             double bs = 0.0;
             for (int t = rp[i]; t < rp[i + 1]; ++t) {
                 const int j = ci[t];
                 if (mate[j] >= 0 || is_blue(j, round)) continue;
-                const double s = csr_strength((int)i, t, rp, ci, vd, dd);
+                const double s = st[t];
                 if (s > 0.0 && s >= (double)theta * smax && (s > bs || (s == bs && j < best))) {
                     bs = s;
                     best = j;
@@ -210,8 +218,8 @@ __global__ __launch_bounds__(FLAT) void k_propose_csr(long n, const int* rp, con
 }
 
 // the matrix pattern is symmetric, so every row that proposed to j is in j's row
-__global__ __launch_bounds__(FLAT) void k_respond_csr(long n, const int* rp, const int* ci, const double* vd,
-                                                      const double* dd, int* mate, const int* prop, int round) {
+__global__ __launch_bounds__(FLAT) void k_respond_csr(long n, const int* rp, const int* ci, const double* st, int* mate,
+                                                      const int* prop, int round) {
     G_STRIDE(j, n) {
         if (mate[j] >= 0 || is_blue(j, round)) continue;
         int best = -1;
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(FLAT) void k_respond_csr(long n, const int* rp, con
         for (int t = rp[j]; t < rp[j + 1]; ++t) {
             const int i = ci[t];
             if (prop[i] != (int)j) continue;
-            const double s = csr_strength((int)j, t, rp, ci, vd, dd);
+            const double s = st[t];
             if (s > 0.0 && (s > bs || (s == bs && i < best))) {
                 bs = s;
                 best = i;
@@ -625,19 +633,23 @@ int pair_pass0(hs_ctx* c, AmgHier* H, const V& v, long n, float theta, long* nag
     return HS_OK;
 }
 
-int pair_pass_csr(hs_ctx* c, AmgHier* H, long n, const int* rp, const int* ci, const double* vd, const double* dd,
-                  float theta, long* nagg, int** agg, int** memb2) {
+int pair_pass_csr(hs_ctx* c, AmgHier* H, long n, long nnz, const int* rp, const int* ci, const double* vd,
+                  const double* dd, float theta, long* nagg, int** agg, int** memb2) {
     HS_TRY(ensure_scratch(H, n));
     int *mate = H->scratch, *prop = mate + n, *flag = prop + n, *pos = flag + n, *bsum = pos + n;
     const int nb = blocks_for(n);
+    double* st = nullptr;
+    HS_TRY(amalloc((void**)&st, std::max(1L, nnz) * sizeof(double)));
+    k_strength_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, st);
     HS_CHECK(hipMemsetAsync(mate, 0xff, n * sizeof(int), c->stream));
     for (int r = 0; r < ROUNDS; ++r) {
-        k_propose_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, mate, prop, theta, r);
-        k_respond_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, mate, prop, r);
+        k_propose_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, st, mate, prop, theta, r);
+        k_respond_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, st, mate, prop, r);
     }
     k_lead_csr<<<nb, FLAT, 0, c->stream>>>(n, mate, flag);
     HS_CHECK(hipGetLastError());
-    HS_TRY(scan(c, H, n, flag, pos, bsum, nagg));
+    HS_TRY(scan(c, H, n, flag, pos, bsum, nagg));   // synchronises: the strengths may go
+    (void)hipFree(st);
     HS_TRY(amalloc((void**)agg, n * sizeof(int)));
     HS_TRY(amalloc((void**)memb2, 2 * std::max(1L, *nagg) * sizeof(int)));
     k_number<<<nb, FLAT, 0, c->stream>>>(n, flag, pos, mate, *agg, *memb2);
@@ -786,7 +798,9 @@ int make_level(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine, float t
     if (nf < 65536) theta = 0.0f;
     int e = HS_OK;
     // pass A over the level above -> intermediate matrix
-    if (fine) e = pair_pass_csr(c, H, nf, fine->rp, fine->ci, fine->vd, fine->dd, theta, &L->ni, &L->aggi, &L->membf);
+    if (fine)
+        e = pair_pass_csr(c, H, nf, fine->nnz, fine->rp, fine->ci, fine->vd, fine->dd, theta, &L->ni, &L->aggi,
+                          &L->membf);
     else e = pair_pass0<V>(c, H, *v0, nf, theta, &L->ni, &L->aggi, &L->membf);
     if (e == HS_OK && L->ni == 0) {   // no unknowns on the level above: nothing to coarsen
         level_free(L);
@@ -794,7 +808,8 @@ int make_level(hs_ctx* c, AmgHier* H, const V* v0, const AmgLevel* fine, float t
     }
     if (e == HS_OK) e = refill_intermediate<V>(c, H, v0, fine, L);
     // pass B over the intermediate matrix -> the new level
-    if (e == HS_OK) e = pair_pass_csr(c, H, L->ni, L->rpi, L->cii, L->vdi, L->ddi, theta, &L->n, &L->aggb, &L->membi);
+    if (e == HS_OK)
+        e = pair_pass_csr(c, H, L->ni, L->nnzi, L->rpi, L->cii, L->vdi, L->ddi, theta, &L->n, &L->aggb, &L->membi);
     if (e == HS_OK) e = refill_level(c, H, L);
     if (e == HS_OK) {
         e = amalloc((void**)&L->agg, std::max(1L, nf) * sizeof(int));
