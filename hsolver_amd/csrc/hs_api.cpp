@@ -158,6 +158,13 @@ int gs_loop(hs_ctx* c, int id, double n, double tol, int maxsteps, int quirk, Sw
         HS_TRY(sweep());
         HS_TRY(after());
     }
+    if (fixed && c->defer_balance && c->h_wdefer) {
+        // the sweep count is fixed: nothing to decide on the host, so no synchronisation
+        // here; hs_step reads the control block with its final one
+        HS_CHECK(hipMemcpyAsync(&c->h_wdefer[id], c->wctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, c->stream));
+        c->wdefer[id] = true;
+        return HS_OK;
+    }
     HS_TRY(read_wctl(c));
     while (c->h_wctl->active && s < maxsteps) {
         HS_TRY(sweep());
@@ -483,6 +490,7 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     if (e == HS_OK) e = dalloc(&c->wctl, 1);
     if (e == HS_OK && hipHostMalloc((void**)&c->h_sctl, sizeof(SolveCtl), 0) != hipSuccess) e = HS_E_NOMEM;
     if (e == HS_OK && hipHostMalloc((void**)&c->h_wctl, sizeof(SweepCtl), 0) != hipSuccess) e = HS_E_NOMEM;
+    if (e == HS_OK && hipHostMalloc((void**)&c->h_wdefer, 5 * sizeof(SweepCtl), 0) != hipSuccess) e = HS_E_NOMEM;
     if (e != HS_OK) { hs_destroy(c); return e; }
 
     // geometry
@@ -554,6 +562,7 @@ void hs_destroy(hs_ctx* c) {
     if (c->if_solid) hipFree(c->if_solid);
     if (c->h_sctl) hipHostFree(c->h_sctl);
     if (c->h_wctl) hipHostFree(c->h_wctl);
+    if (c->h_wdefer) hipHostFree(c->h_wdefer);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -688,6 +697,19 @@ int hs_step(hs_ctx* c, hs_stats* st) {
         loc.heat_in = heat;
         loc.energy_residual = std::fabs(res) / std::max(std::fabs(heat), 1e-300);
     }
+    // the fixed-count sweep loops' deferred control blocks (also read by set_for_next's sync)
+    bool nan = false;
+    for (int id = 0; id < 5; ++id) {
+        if (!c->wdefer[id]) continue;
+        c->wdefer[id] = false;
+        const SweepCtl& w = c->h_wdefer[id];
+        c->gs_last[id] = w.step;
+        if (id < 3) { loc.mom_sweeps[id] = w.step; loc.mom_residual[id] = w.global; }
+        else if (id == 3) { loc.t_sweeps = w.step; loc.t_residual = w.global; }
+        else { loc.ts_sweeps = w.step; loc.ts_residual = w.global; }
+        nan = nan || std::isnan(w.global);
+    }
+    if (nan) return HS_E_NAN;
     if (st) *st = loc;
     return HS_OK;
 }

@@ -221,6 +221,11 @@ struct hs_ctx {
     int kx_hist = 0;
     double* khist[2] = {nullptr, nullptr};
     bool defer_balance = false;  // hs_step reads the energy balance at its final synchronisation
+    // fixed-count sweep loops inside hs_step (Mode C: no tolerance) queue their control
+    // block into h_wdefer[id] and hs_step reads the counts and residuals at its final
+    // synchronisation; wdefer[id] marks a slot to read
+    SweepCtl* h_wdefer = nullptr;       // pinned, 5 slots (momentum x/y/z, energy, solid)
+    bool wdefer[5] = {false, false, false, false, false};
 
     // host copies
     std::vector<double> X, Y, Z;
