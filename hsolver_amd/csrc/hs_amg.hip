@@ -75,6 +75,12 @@ struct PView {   // p' operator on the pressure multigrid's level 0 (symmetric)
     }
     __device__ const T* x() const { return L.x; }
     __device__ const T* b() const { return L.b; }
+    // the same tests for a cell (i, j, k) of the interior and its neighbour in direction d
+    __device__ bool active_at(long c, int, int, int) const { return L.act[c] != 0; }
+    __device__ bool nb_active(long nb, int, int, int, int) const { return L.act[nb] != 0; }
+    __device__ int nx() const { return L.nx; }
+    __device__ int ny() const { return L.ny; }
+    __device__ int nz() const { return L.nz; }
 };
 
 template <typename T>
@@ -82,9 +88,25 @@ struct EView {   // energy operator A (tr = 0) or A^T (tr = 1) on the energy lev
     heo::EL<T> L;
     int tr;
     __device__ bool active(long c) const {
-        const long i = c % L.sy, j = (c / L.sy) % (L.ny + 2), k = c / L.sz;
-        return i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
+        // 32-bit decoding (the padded layout has fewer than 2^32 cells, checked at setup)
+        const unsigned cu = (unsigned)c, sy = (unsigned)L.sy, ny2 = (unsigned)(L.ny + 2);
+        const unsigned q = cu / sy, i = cu - q * sy, k = q / ny2, j = q - k * ny2;
+        return i >= 1 && i <= (unsigned)L.nx && j >= 1 && j <= (unsigned)L.ny && k >= 1 && k <= (unsigned)L.nz;
     }
+    __device__ bool active_at(long, int, int, int) const { return true; }   // every interior cell
+    __device__ bool nb_active(long, int d, int i, int j, int k) const {
+        switch (d) {
+            case 0: return i > 1;
+            case 1: return i < L.nx;
+            case 2: return j > 1;
+            case 3: return j < L.ny;
+            case 4: return k > 1;
+            default: return k < L.nz;
+        }
+    }
+    __device__ int nx() const { return L.nx; }
+    __device__ int ny() const { return L.ny; }
+    __device__ int nz() const { return L.nz; }
     __device__ long off(int d) const {
         const long s = d < 2 ? 1 : (d < 4 ? L.sy : L.sz);
         return (d & 1) ? s : -s;
@@ -511,25 +533,28 @@ __global__ __launch_bounds__(FLAT) void k_resid(long n, const int* so, const int
     G_STRIDE(i, n) r[i] = b[i] - dg[i] * x[i] - sell_offsum(i, so, sw, sc, sv, x);
 }
 
-// r = b - A x on the structured finest level (0 on rows that are not unknowns)
+// r = b - A x on the interior of the structured finest level (0 on cells that are not
+// unknowns; the ghost layer is not written: only aggregate members are read back), one
+// thread per cell of an x row, rows by block (y, z)
 template <typename V, typename T>
-__global__ __launch_bounds__(FLAT) void k_resid0(V v, long n, T* r, const int* skip) {
+__global__ __launch_bounds__(64) void k_resid0(V v, long sy, long sz, T* r, const int* skip) {
     if (*skip) return;
+    const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > v.nx()) return;
+    const long c = i + sy * j + sz * k;
     const T* x = v.x();
     const T* b = v.b();
-    G_STRIDE(c, n) {
-        T e = T(0);
-        if (v.active(c)) {
-            double a[6];
-            v.coup(c, a);
-            e = b[c] - (T)v.diag(c) * x[c];
-            for (int d = 0; d < 6; ++d) {
-                const long nb = c + v.off(d);
-                if (v.active(nb)) e += (T)a[d] * x[nb];
-            }
+    T e = T(0);
+    if (v.active_at(c, i, j, k)) {
+        double a[6];
+        v.coup(c, a);
+        e = b[c] - (T)v.diag(c) * x[c];
+        for (int d = 0; d < 6; ++d) {
+            const long nb = c + v.off(d);
+            if (v.nb_active(nb, d, i, j, k)) e += (T)a[d] * x[nb];
         }
-        r[c] = e;
     }
+    r[c] = e;
 }
 
 // bc[I] = sum of r over the members of I
@@ -855,6 +880,7 @@ AmgOpt options(const hs_ctx* c, bool energy) {
 }
 
 int hier_alloc(AmgHier** Hp, bool f32, int kind, long n0) {
+    if (n0 >= 2147483647L) return HS_E_ARG;   // row numbers are ints
     AmgHier* H = *Hp;
     if (!H) {
         H = new AmgHier();
@@ -957,9 +983,10 @@ int cycle_csr(hs_ctx* c, AmgHier* H, size_t l, const AmgOpt& o, const int* skip)
 // free between the smoother's sweeps), gathered onto the first aggregated level, the
 // cycle there, the scaled correction added to x0
 template <typename T, typename V>
-int coarse_part(hs_ctx* c, AmgHier* H, const V& v0, T* x0, T* r0, const AmgOpt& o, const int* skip) {
+int coarse_part(hs_ctx* c, AmgHier* H, const V& v0, int nx, int ny, int nz, long sy, long sz, T* x0, T* r0,
+                const AmgOpt& o, const int* skip) {
     AmgLevel* L1 = H->lv[0];
-    k_resid0<V, T><<<blocks_for(H->n0), FLAT, 0, c->stream>>>(v0, H->n0, r0, skip);
+    k_resid0<V, T><<<dim3((nx + 63) / 64, ny, nz), 64, 0, c->stream>>>(v0, sy, sz, r0, skip);
     k_gather4<T><<<blocks_for(L1->n), FLAT, 0, c->stream>>>(L1->n, L1->memb, r0, (T*)L1->b, skip);
     HS_CHECK(hipGetLastError());
     HS_TRY(cycle_csr<T>(c, H, 0, o, skip));
@@ -1007,10 +1034,10 @@ int amg_coarse_p(hs_ctx* c, const int* skip) {
     const AmgOpt o = options(c, false);
     if (H->f32) {
         const PView<float> v{mgl::lv<float>(m)};
-        return coarse_part<float, PView<float>>(c, H, v, (float*)m->x, (float*)m->x2, o, skip);
+        return coarse_part<float, PView<float>>(c, H, v, m->nx, m->ny, m->nz, m->sy, m->sz, (float*)m->x, (float*)m->x2, o, skip);
     }
     const PView<double> v{mgl::lv<double>(m)};
-    return coarse_part<double, PView<double>>(c, H, v, (double*)m->x, (double*)m->x2, o, skip);
+    return coarse_part<double, PView<double>>(c, H, v, m->nx, m->ny, m->nz, m->sy, m->sz, (double*)m->x, (double*)m->x2, o, skip);
 }
 
 // energy: one aggregation per operator build and direction (the operator is frozen for
@@ -1044,10 +1071,10 @@ int amg_coarse_e(hs_ctx* c, ELevel* l0, int tr, const int* skip) {
     const AmgOpt o = options(c, true);
     if (H->f32) {
         const EView<float> v{heo::el<float>(l0), tr};
-        return coarse_part<float, EView<float>>(c, H, v, (float*)l0->x, (float*)l0->x2, o, skip);
+        return coarse_part<float, EView<float>>(c, H, v, l0->nx, l0->ny, l0->nz, l0->sy, l0->sz, (float*)l0->x, (float*)l0->x2, o, skip);
     }
     const EView<double> v{heo::el<double>(l0), tr};
-    return coarse_part<double, EView<double>>(c, H, v, (double*)l0->x, (double*)l0->x2, o, skip);
+    return coarse_part<double, EView<double>>(c, H, v, l0->nx, l0->ny, l0->nz, l0->sy, l0->sz, (double*)l0->x, (double*)l0->x2, o, skip);
 }
 
 void amg_free(AmgHier** Hp) {
