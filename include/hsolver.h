@@ -330,7 +330,8 @@ int hs_energy_solve_rhs(hs_ctx* c, int32_t transpose, const double* b, double* x
                         int32_t maxit, int32_t mem, hs_solve_info* info);
 /* y = A x (transpose = 0) or A^T x (transpose = 1) of the energy operator. */
 int hs_energy_apply(hs_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem);
-/* Global energy balance of the current temperature field (Mode C): *heat_in = sum q V,
+/* Global energy balance of the current temperature field (Mode C; no counterpart in the
+ * reference, which computes no balance): *heat_in = sum q V,
  * *residual = |sum over cells of the energy equation's residual| / |heat_in|, the figure
  * hs_stats.energy_residual reports after each hs_step. */
 int hs_energy_balance(hs_ctx* c, double* heat_in, double* residual);
@@ -354,7 +355,8 @@ int hs_prof_get(hs_ctx* c, int64_t* count, double* total_ms);
 int hs_copy_bench(int64_t nbytes, int32_t reps, double* avg_ms);
 /* multigrid level dimensions (3 ints per level); returns the number of levels */
 int hs_mg_levels(hs_ctx* c, int32_t* dims, int32_t max_levels);
-/* aggregation multigrid levels below the finest one (which = 0: the p' hierarchy, 1: the
+/* aggregation multigrid levels (no counterpart in the reference: HYPRE keeps BoomerAMG's
+ * hierarchy internal) below the finest one (which = 0: the p' hierarchy, 1: the
  * energy hierarchy): rows and stored off-diagonal entries, 2 per level, coarsest last;
  * returns the number of levels (0 before the first build) */
 int hs_amg_levels(hs_ctx* c, int32_t which, int64_t* rows_nnz, int32_t max_levels);
