@@ -472,6 +472,7 @@ int hs_create(const hs_grid* g, const hs_params* p, hs_ctx** out) {
     A(&c->u, d.nU); A(&c->up, d.nU); A(&c->v, d.nV); A(&c->vp, d.nV); A(&c->w, d.nW); A(&c->wp, d.nW);
     A(&c->p, d.nC); A(&c->psnap, d.nC); A(&c->q, d.nC); A(&c->k, d.nC);
     A(&c->ptemp, d.nP); A(&c->pprime, d.nP); A(&c->tf, d.nP); A(&c->ts, d.nP); A(&c->b, d.nP);
+    A(&c->qsum, 1);
     if (modeR(c)) { A(&c->ax, d.nP); A(&c->ay, d.nP); A(&c->az, d.nP); }
     else { A(&c->ax, d.nU); A(&c->ay, d.nV); A(&c->az, d.nW); }
     A(&c->kb, d.nP); A(&c->kr, d.nP); A(&c->kp, d.nP); A(&c->kp2, d.nP); A(&c->ks, d.nP); A(&c->kx, d.nP); A(&c->kz, d.nP); A(&c->kq, d.nP);
@@ -550,7 +551,7 @@ void hs_destroy(hs_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     gmg_free(c);
     energy_free(c);
-    double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->k, c->ptemp,
+    double* bufs[] = {c->geo, c->u, c->up, c->v, c->vp, c->w, c->wp, c->p, c->psnap, c->q, c->qsum, c->k, c->ptemp,
                       c->pprime, c->tf, c->ts, c->b, c->ax, c->ay, c->az, c->kr, c->kp, c->kp2, c->ks, c->kx,
                       c->kz, c->kq, c->kb, c->dres, c->partials, c->khist[0], c->khist[1]};
     for (double* b : bufs) if (b) hipFree(b);
@@ -759,6 +760,7 @@ int hs_put_array(hs_ctx* c, int32_t which, const double* host, int64_t n) {
     const int64_t s = array_len(c, which);
     if (s < 0 || n != s) return HS_E_ARG;
     HS_CHECK(hipMemcpyAsync(array_ptr(c, which), host, s * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (which == HS_A_Q) ++c->q_version;
     HS_CHECK(hipStreamSynchronize(c->stream));
     return HS_OK;
 }
@@ -973,6 +975,7 @@ int hs_put_cells(hs_ctx* c, int32_t which, const double* src, int32_t mem) {
     const hipMemcpyKind kind = mem == HS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (which == HS_A_Q || which == HS_A_K) {
         HS_CHECK(hipMemcpyAsync(which == HS_A_Q ? c->q : c->k, src, c->d.nC * sizeof(double), kind, c->stream));
+        if (which == HS_A_Q) ++c->q_version;
     } else if (which == HS_A_TF) {
         double* s1;
         HS_TRY(energy_stage(c, &s1, nullptr, nullptr));

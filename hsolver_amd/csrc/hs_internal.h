@@ -245,6 +245,10 @@ struct hs_ctx {
     double *u = nullptr, *v = nullptr, *w = nullptr, *up = nullptr, *vp = nullptr, *wp = nullptr;
     double *p = nullptr, *psnap = nullptr, *ptemp = nullptr, *pprime = nullptr, *tf = nullptr, *ts = nullptr;
     double *q = nullptr, *k = nullptr, *b = nullptr;
+    // the heat input sum q V of the energy balance, kept on the device and recomputed only
+    // after q changed (q_version counts the writes of q)
+    double* qsum = nullptr;
+    long q_version = 0, qsum_version = -1;
     // p' operator: Mode R per-cell row coefficients (P), Mode C per-face (U/V/W)
     double *ax = nullptr, *ay = nullptr, *az = nullptr;
     // Krylov work (P layout)

@@ -1290,9 +1290,14 @@ static int energy_balance_launch(hs_ctx* c) {
     k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials);
     HS_CHECK(hipGetLastError());
     HS_TRY(launch_finish(c, FIN_DOT, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
-    k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials);
-    HS_CHECK(hipGetLastError());
-    return launch_finish(c, FIN_DOT, FLAT_BLOCKS, &c->sctl->red[2]);
+    if (c->qsum_version != c->q_version) {   // q changed since the last sum (or never summed)
+        k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials);
+        HS_CHECK(hipGetLastError());
+        HS_TRY(launch_finish(c, FIN_DOT, FLAT_BLOCKS, c->qsum));
+        c->qsum_version = c->q_version;
+    }
+    HS_CHECK(hipMemcpyAsync(&c->sctl->red[2], c->qsum, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return HS_OK;
 }
 
 int energy_balance_async(hs_ctx* c) {
