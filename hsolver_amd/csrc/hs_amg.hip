@@ -39,6 +39,7 @@
 // Galerkin values are recomputed, until the dense-inverse policy of the geometric path
 // (mg_dense_coarse builds, or an FCG iteration rise of 2) asks for a fresh aggregation.
 #include <cmath>
+#include <cstring>
 
 #include "hs_device.h"
 #include "hs_eop.h"
@@ -299,9 +300,9 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_local(long n, const int* in, in
     if (threadIdx.x == SCAN_T - 1) bsum[blockIdx.x] = sh[SCAN_T - 1];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int nb, int* bsum, int* total) {
+__global__ __launch_bounds__(1024) void k_scan_top(int nb, int* bsum, long long* total) {
     __shared__ int sh[1024];
-    __shared__ int carry;
+    __shared__ long long carry;   // 64-bit, so that a total beyond the int range is seen
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < nb; base += 1024) {
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nb, int* bsum, int* total
             sh[threadIdx.x] += t;
             __syncthreads();
         }
-        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+        if (i < nb) bsum[i] = (int)(carry + sh[threadIdx.x] - v);
         __syncthreads();
         if (threadIdx.x == 1023) carry += sh[1023];
         __syncthreads();
@@ -626,12 +627,15 @@ int scan(hs_ctx* c, AmgHier* H, long n, const int* in, int* out, int* bsum, long
     }
     const int nb = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
     k_scan_local<<<nb, SCAN_T, 0, c->stream>>>(n, in, out, bsum);
-    k_scan_top<<<1, 1024, 0, c->stream>>>(nb, bsum, H->dflag + 1);
+    k_scan_top<<<1, 1024, 0, c->stream>>>(nb, bsum, reinterpret_cast<long long*>(H->dflag + 2));
     k_scan_add<<<blocks_for(n), FLAT, 0, c->stream>>>(n, out, bsum);
     HS_CHECK(hipGetLastError());
-    HS_CHECK(hipMemcpyAsync(H->h_flag, H->dflag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HS_CHECK(hipMemcpyAsync(H->h_flag, H->dflag, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
-    *total = H->h_flag[1];
+    long long t;
+    std::memcpy(&t, H->h_flag + 2, sizeof t);
+    if (t > 2147483647LL) return HS_E_ARG;   // offsets are ints: a level this large is not built
+    *total = (long)t;
     return HS_OK;
 }
 
@@ -663,8 +667,12 @@ int pair_pass_csr(hs_ctx* c, AmgHier* H, long n, long nnz, const int* rp, const 
     HS_TRY(ensure_scratch(H, n));
     int *mate = H->scratch, *prop = mate + n, *flag = prop + n, *pos = flag + n, *bsum = pos + n;
     const int nb = blocks_for(n);
-    double* st = nullptr;
-    HS_TRY(amalloc((void**)&st, std::max(1L, nnz) * sizeof(double)));
+    struct Owned {   // the strengths, freed on every return
+        double* p = nullptr;
+        ~Owned() { if (p) (void)hipFree(p); }
+    } own;
+    HS_TRY(amalloc((void**)&own.p, std::max(1L, nnz) * sizeof(double)));
+    double* st = own.p;
     k_strength_csr<<<nb, FLAT, 0, c->stream>>>(n, rp, ci, vd, dd, st);
     HS_CHECK(hipMemsetAsync(mate, 0xff, n * sizeof(int), c->stream));
     for (int r = 0; r < ROUNDS; ++r) {
@@ -674,7 +682,8 @@ int pair_pass_csr(hs_ctx* c, AmgHier* H, long n, long nnz, const int* rp, const 
     k_lead_csr<<<nb, FLAT, 0, c->stream>>>(n, mate, flag);
     HS_CHECK(hipGetLastError());
     HS_TRY(scan(c, H, n, flag, pos, bsum, nagg));   // synchronises: the strengths may go
-    (void)hipFree(st);
+    (void)hipFree(own.p);
+    own.p = nullptr;
     HS_TRY(amalloc((void**)agg, n * sizeof(int)));
     HS_TRY(amalloc((void**)memb2, 2 * std::max(1L, *nagg) * sizeof(int)));
     k_number<<<nb, FLAT, 0, c->stream>>>(n, flag, pos, mate, *agg, *memb2);
@@ -871,10 +880,10 @@ struct AmgOpt {
 AmgOpt options(const hs_ctx* c, bool energy) {
     const hs_params& p = c->prm;
     AmgOpt o;
-    o.theta = p.amg_theta;     // likewise
+    o.theta = p.amg_theta;     // chosen at hs_create when left <= 0
     o.alpha = p.amg_alpha > 0.0f ? (double)p.amg_alpha : (energy ? 1.6 : 1.4);
-    o.sweeps = p.amg_sweeps;   // chosen at hs_create when left <= 0
-    o.refine = std::max(1, p.mg_dense_refine);
+    o.sweeps = p.amg_sweeps;   // likewise
+    o.refine = std::max(0, p.mg_dense_refine);
     o.coarse = std::max(1, p.mg_coarse_sweeps);
     return o;
 }

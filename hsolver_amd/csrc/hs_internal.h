@@ -202,7 +202,7 @@ struct AmgHier {
     DenseCoarse* dense = nullptr;   // inverse of the coarsest level when it is small enough
     int* scratch = nullptr;         // matching / numbering work (ints)
     long scratch_n = 0;
-    int* dflag = nullptr;           // device: [0] row-width overflow, [1] scan total
+    int* dflag = nullptr;           // device: [0] row-width overflow, [2..3] scan total (64-bit)
     int* h_flag = nullptr;          // pinned mirror
     long builds = 0, aggregations = 0, since = 0;
     int iters_ref = -1;             // p' FCG iterations of the first solve after an aggregation
