@@ -84,6 +84,8 @@ struct SolveCtl {
     int    nan;        // 1: a Krylov reduction came out NaN or Inf (the solve stopped there)
     int    cold;       // 1: a warm start's residual was not below ||b||: start from zero instead
     double red[8];     // generic reduction results
+    double cg_g[2];    // unpreconditioned CG: <r_k, r_k> in slot k & 1 (a kernel that forms
+                       // the next one writes the other slot while its blocks read this one)
 };
 
 struct SweepCtl {
@@ -293,8 +295,8 @@ int prof_end(hs_ctx* c, int slot);
 enum FinishKind : int {
     FIN_SWEEP = 0,      // Gauss-Seidel loop control (wctl)
     FIN_CG_SETUP,       // <b,b>
-    FIN_CG_ALPHA,       // <p,Ap> -> alpha, breakdown checks
-    FIN_CG_UPDATE,      // <r,r> -> HYPRE stop rule, beta
+    FIN_CG_ALPHA,       // <p,Ap> -> alpha, breakdown checks (flexible CG's first apply)
+    FIN_CG_UPDATE,      // <r,r> of CG update `it` -> HYPRE stop rule, beta (end of a batch)
     FIN_DOT,            // plain sum -> *out
     FIN_FCG_SETUP,      // <b,b>
     FIN_FCG_START,      // <r,z> -> gamma
@@ -306,6 +308,8 @@ enum FinishKind : int {
 
 // ---- kernel launchers (defined in the .hip files) ----
 int launch_finish(hs_ctx* c, int kind, int n, double* out);
+// FIN_CG_UPDATE over the n partials at part, for CG update number it (0-based)
+int launch_finish_cg(hs_ctx* c, const double* part, int n, int it);
 int launch_fill(hs_ctx* c, double* a, long n, double v);
 int launch_copy(hs_ctx* c, double* dst, const double* src, long n);
 int bc_x_ref(hs_ctx* c);

@@ -655,6 +655,101 @@ __device__ __forceinline__ void spmv_column(const OpArgs& o, XA x, long i, long 
     }
 }
 
+// Sum of n block partials with the same bits in every block that calls it with the same
+// NT: thread t < NT adds entries t, t + NT, ... (eight loads issued together per round),
+// each wave adds its lanes, and every thread adds the NT / 64 wave sums in order.  Every
+// thread of the block calls it (it holds one barrier); threads t >= NT add nothing.
+// *flag: thread 0's value is handed to every thread through the same barrier.
+template <int NT>
+__device__ __forceinline__ double sum_partials_all(const double* part, int n, int tid, int* flag) {
+    __shared__ double sh[NT / 64];
+    __shared__ int sh_flag;
+    if (tid < NT) {
+        double s = 0.0;
+        for (int base = 0; base < n; base += 8 * NT) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int t = base + tid + NT * k;
+                v[k] = t < n ? part[t] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s += v[k];
+        }
+        s = hsd::wave_sum(s);
+        if ((tid & 63) == 0) sh[tid >> 6] = s;
+        if (tid == 0) sh_flag = *flag;
+    }
+    __syncthreads();
+    *flag = sh_flag;
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) r += sh[w];
+    return r;
+}
+
+// Scalar steps of HYPRE's PCG with the identity preconditioner (fluid_solver.h:1789-1790,
+// settings :1871-1875).  gamma_k = <r_k, r_k> is kept in ctl->cg_g[k & 1], so a kernel
+// whose blocks all form gamma_{k+1} from the same partials writes the other slot while
+// they still read gamma_k.  Each step is a pure function of the control block and the
+// reduced value; block 0 of the kernel that takes it (or the one-block finish) records it.
+struct CgStep {
+    double v;     // alpha or beta
+    int stop;     // the solve ends here (breakdown, convergence, maxit, non-finite)
+    int nan;
+    int conv;
+    int iters;
+};
+
+// the control-block values a CG step reads, loaded at a kernel's start together with the
+// partials (none of them is written while the kernel runs, apart from done)
+struct CgIn {
+    double bb, eps, g;    // <b,b>, tol^2, gamma_it
+    int maxit, done;
+};
+__device__ __forceinline__ CgIn cg_in(const SolveCtl* ctl, int it) {
+    return CgIn{ctl->bb, ctl->eps, ctl->cg_g[it & 1], ctl->maxit, ctl->done};
+}
+
+// alpha of CG iteration it from <p, A p>: a zero or non-finite <p, A p>, or alpha not
+// above DBL_MIN, ends the solve (HYPRE's breakdown exits)
+__device__ __forceinline__ CgStep cg_alpha_step(const CgIn& ci, double pap) {
+    CgStep s{0.0, 0, 0, 0, 0};
+    if (!isfinite(pap)) { s.stop = 1; s.nan = 1; return s; }
+    if (pap == 0.0) { s.stop = 1; return s; }
+    s.v = ci.g / pap;
+    if (!(s.v > DBL_MIN)) s.stop = 1;
+    return s;
+}
+__device__ __forceinline__ void cg_alpha_record(SolveCtl* ctl, double pap, const CgStep& s) {
+    if (s.nan) { ctl->nan = 1; ctl->done = 1; ctl->converged = 0; return; }
+    ctl->sdotp = pap;
+    if (s.stop) { ctl->done = 1; return; }
+    ctl->alpha = s.v;
+}
+
+// after CG update it (<r,r> = rr): iteration count it + 1, the relative stop rule
+// <r,r> / <b,b> < tol^2, maxit, and beta = gamma_{it+1} / gamma_it
+__device__ __forceinline__ CgStep cg_update_step(const CgIn& ci, double rr, int it) {
+    CgStep s{0.0, 0, 0, 0, it + 1};
+    if (!isfinite(rr)) { s.stop = 1; s.nan = 1; return s; }
+    if (rr / ci.bb < ci.eps) { s.stop = 1; s.conv = 1; return s; }
+    if (s.iters >= ci.maxit) { s.stop = 1; return s; }
+    s.v = rr / ci.g;
+    return s;
+}
+__device__ __forceinline__ void cg_update_record(SolveCtl* ctl, const CgIn& ci, double rr, int it, const CgStep& s) {
+    if (s.nan) { ctl->nan = 1; ctl->done = 1; ctl->converged = 0; return; }
+    ctl->iters = s.iters;
+    ctl->gamma_old = ci.g;
+    ctl->cg_g[(it + 1) & 1] = rr;
+    ctl->gamma = rr;
+    ctl->rr = rr;
+    if (s.conv) ctl->converged = 1;
+    if (s.stop) { ctl->done = 1; return; }
+    ctl->beta = s.v;
+}
+
 // s = A p (z-marching columns) with block partials of <s,p> (k_finish: alpha).
 __global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, double* s, int kc, int gx, int gy, int gz,
                                                 double* partials, SolveCtl* ctl) {
@@ -681,14 +776,32 @@ __global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, doubl
 // formed at every point the stencil reads (the same expression as a separate update pass,
 // so the same values), written to pn for the column's cells, and q = A p_new with block
 // partials of <p_new, q>.  pn must not alias p (neighbouring columns still read p_old).
+// With fold (plain CG, z = r), beta is not read from ctl: every block takes the stop rule
+// and beta of CG update fold_it itself from that update's nfold block partials of <r,r>
+// (cg_update_step), block 0 records them, and a finished solve returns here.
 template <typename T>
 __global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__ z, const double* __restrict__ p,
                                                  double* __restrict__ pn, double* __restrict__ s, int kc, int gx, int gy,
-                                                 int gz, double* partials, SolveCtl* ctl) {
-    if (ctl->done) return;
-    const Dims& d = o.d;
-    const double beta = ctl->beta;
+                                                 int gz, double* partials, SolveCtl* ctl, const double* fold, int nfold,
+                                                 int fold_it) {
     const int tid = threadIdx.x + BX * threadIdx.y;
+    double beta;
+    if (fold) {
+        // thread 0's done flag is handed on through the sum's barrier, so the block takes
+        // one decision
+        const CgIn ci = cg_in(ctl, fold_it);
+        int done = ci.done;
+        const double rr = sum_partials_all<BT>(fold, nfold, tid, &done);
+        if (done) return;
+        const CgStep st = cg_update_step(ci, rr, fold_it);
+        if (blockIdx.x == 0 && tid == 0) cg_update_record(ctl, ci, rr, fold_it, st);
+        if (st.stop) return;
+        beta = st.v;
+    } else {
+        if (ctl->done) return;
+        beta = ctl->beta;
+    }
+    const Dims& d = o.d;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
     const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
     double acc = 0.0;
@@ -705,12 +818,23 @@ __global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__
     put_partial<BT>(acc, tid, blockIdx.x, partials);
 }
 
-// x += alpha p, r -= alpha s, block partials of <r,r> (k_finish: stop rule, beta)
-__global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* x, double* r, const double* p, const double* s,
-                                                    double* partials, SolveCtl* ctl) {
-    if (ctl->done) return;
-    const double alpha = ctl->alpha;
+// x += alpha p, r -= alpha s, block partials of <r,r> (the next apply or k_finish: stop
+// rule, beta).  alpha of CG iteration it is formed by every block alike from the apply's
+// nsp block partials of <p, A p> (cg_alpha_step); block 0 records it.
+__global__ __launch_bounds__(FLAT) void k_cg_update(long n, double* __restrict__ x, double* __restrict__ r,
+                                                    const double* __restrict__ p, const double* __restrict__ s,
+                                                    const double* sp, int nsp, int it, double* partials,
+                                                    SolveCtl* ctl) {
+    const CgIn ci = cg_in(ctl, it);
+    int done = ci.done;
+    const double pap = sum_partials_all<FLAT>(sp, nsp, threadIdx.x, &done);
+    if (done) return;
+    const CgStep st = cg_alpha_step(ci, pap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cg_alpha_record(ctl, pap, st);
+    if (st.stop) return;
+    const double alpha = st.v;
     double acc = 0.0;
+#pragma unroll 4
     for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < n; t += (long)gridDim.x * FLAT) {
         x[t] += alpha * p[t];
         const double rv = r[t] - alpha * s[t];
@@ -743,7 +867,7 @@ __global__ __launch_bounds__(FLAT) void k_cg_setup(long n, const double* b, doub
 // the scalar step of the kernel that produced them.
 
 __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, int kind, SolveCtl* ctl, SweepCtl* wctl,
-                                                 double* out) {
+                                                 double* out, int it) {
     if (kind == FIN_SWEEP) {
         if (!wctl->active) return;
     } else if (kind == FIN_CG_ALPHA || kind == FIN_CG_UPDATE || kind == FIN_FCG_START || kind == FIN_FCG_UPDATE ||
@@ -751,7 +875,11 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
         if (ctl->done) return;
     }
     const int tid = threadIdx.x;
-    const double a = hsd::sum_partials<1024>(partials, n, tid);
+    // CG's <r,r> is summed in the order the vector kernels use for it (sum_partials_all
+    // over 256 threads), so a batch boundary does not change a bit of the iterates
+    int unused = 0;
+    const double a = kind == FIN_CG_UPDATE ? sum_partials_all<BT>(partials, n, tid, &unused)
+                                           : hsd::sum_partials<1024>(partials, n, tid);
     double b = 0.0;
     if (kind == FIN_FCG_BETA || kind == FIN_DOT2 || kind == FIN_FCG_SETUP_WARM)
         b = hsd::sum_partials<1024>(partials + n, n, tid);
@@ -772,7 +900,7 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
     switch (kind) {
         case FIN_SWEEP: sweep_control(wctl, a); break;
         case FIN_CG_SETUP:
-            ctl->bb = a; ctl->gamma = a; ctl->rr = a; ctl->iters = 0;
+            ctl->bb = a; ctl->gamma = a; ctl->cg_g[0] = a; ctl->rr = a; ctl->iters = 0;
             ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
             break;
         case FIN_CG_ALPHA: {
@@ -783,12 +911,11 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
             ctl->alpha = alpha;
             break;
         }
-        case FIN_CG_UPDATE:
-            ctl->iters += 1; ctl->gamma_old = ctl->gamma; ctl->gamma = a; ctl->rr = a;
-            if (a / ctl->bb < ctl->eps) { ctl->converged = 1; ctl->done = 1; break; }
-            if (ctl->iters >= ctl->maxit) { ctl->done = 1; break; }
-            ctl->beta = ctl->gamma / ctl->gamma_old;
+        case FIN_CG_UPDATE: {
+            const CgIn ci = cg_in(ctl, it);
+            cg_update_record(ctl, ci, a, it, cg_update_step(ci, a, it));
             break;
+        }
         case FIN_DOT: *out = a; break;
         case FIN_DOT2: out[0] = a; out[1] = b; break;
         case FIN_FCG_SETUP:
@@ -975,7 +1102,12 @@ OpArgs op_args(hs_ctx* c) {
 }  // namespace
 
 int launch_finish(hs_ctx* c, int kind, int n, double* out) {
-    k_finish<<<1, 1024, 0, c->stream>>>(c->partials, n, kind, c->sctl, c->wctl, out);
+    k_finish<<<1, 1024, 0, c->stream>>>(c->partials, n, kind, c->sctl, c->wctl, out, 0);
+    HS_CHECK(hipGetLastError());
+    return HS_OK;
+}
+int launch_finish_cg(hs_ctx* c, const double* part, int n, int it) {
+    k_finish<<<1, 1024, 0, c->stream>>>(part, n, FIN_CG_UPDATE, c->sctl, c->wctl, nullptr, it);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
@@ -1203,23 +1335,35 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resi
     int batch = c->last_iters > 8 ? c->last_iters - 4 : 8;
     // p = b for the first direction (k_cg_setup); later ones, p = r + beta p, are formed
     // inside the apply from the old direction and written to the second buffer (the same
-    // arithmetic as a separate update pass), and the buffers swap
+    // arithmetic as a separate update pass), and the buffers swap.  The scalar steps run
+    // inside the vector kernels: the update forms alpha from the apply's partials, and the
+    // next apply forms the stop rule and beta from the update's partials, so an iteration
+    // is two launches.  The update's partials sit past the apply's; the batch's last
+    // update is finished by k_finish so the host reads a settled control block.
     double *p = c->kp, *p2 = c->kp2;
+    const int nsp = (int)(gr.x * gr.y * gr.z);
+    double* sp = c->partials;
+    double* up = c->partials + std::max(nsp, FLAT_BLOCKS);
+    // update blocks: four cells per thread up to 1024 blocks, so the next apply adds at
+    // most 1024 partials
+    const int nub = (int)std::min<long>(1024, std::max<long>(1, (n + 4 * FLAT - 1) / (4 * FLAT)));
     while (true) {
-        for (int it = 0; it < batch && launched < c->prm.p_maxit; ++it, ++launched) {
+        int in_batch = 0;
+        for (; in_batch < batch && launched < c->prm.p_maxit; ++in_batch, ++launched) {
+            const int slot = prof_begin(c, HS_K_SPMV);
             if (launched == 0) {
-                const int slot = prof_begin(c, HS_K_SPMV);
-                k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, p, c->ks, kc, gr.x, gr.y, gr.z, c->partials,
-                                                                     c->sctl);
-                HS_TRY(prof_end(c, slot));
-                HS_TRY(launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr));
+                k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, p, c->ks, kc, gr.x, gr.y, gr.z, sp, c->sctl);
             } else {
-                HS_TRY(fcg_spmv_dot(c, c->kr, false, p, p2, c->ks));
+                // the batch's first apply reads beta from the control block (k_finish)
+                k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(
+                    o, c->kr, p, p2, c->ks, kc, gr.x, gr.y, gr.z, sp, c->sctl, in_batch == 0 ? nullptr : up, nub,
+                    launched - 1);
                 std::swap(p, p2);
             }
-            k_cg_update<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(n, x, c->kr, p, c->ks, c->partials, c->sctl);
-            HS_TRY(launch_finish(c, FIN_CG_UPDATE, FLAT_BLOCKS, nullptr));
+            HS_TRY(prof_end(c, slot));
+            k_cg_update<<<nub, FLAT, 0, c->stream>>>(n, x, c->kr, p, c->ks, sp, nsp, launched, up, c->sctl);
         }
+        if (in_batch > 0) HS_TRY(launch_finish_cg(c, up, nub, launched - 1));
         HS_CHECK(hipGetLastError());
         if (with_resid) HS_TRY(launch_resid_norms(c, b, x));
         HS_CHECK(hipMemcpyAsync(h, c->sctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
@@ -1332,10 +1476,10 @@ int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* p
     const int slot = prof_begin(c, HS_K_SPMV);
     if (zf32)
         k_fcg_spmv<float><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const float*)z, p, pn, q, kc, gr.x,
-                                                                      gr.y, gr.z, c->partials, c->sctl);
+                                                                      gr.y, gr.z, c->partials, c->sctl, nullptr, 0, 0);
     else
         k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const double*)z, p, pn, q, kc, gr.x,
-                                                                       gr.y, gr.z, c->partials, c->sctl);
+                                                                       gr.y, gr.z, c->partials, c->sctl, nullptr, 0, 0);
     HS_CHECK(hipGetLastError());
     HS_TRY(prof_end(c, slot));
     return launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr);
