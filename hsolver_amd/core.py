@@ -240,7 +240,7 @@ class Context:
 
     def energy_balance(self) -> tuple:
         """(heat input sum q V, |sum of energy residuals| / heat input) of the current field."""
-        fn = self.lib.hs_energy_balance
+        fn = self._fn("energy_balance")
         fn.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         h, r = C.c_double(), C.c_double()
         self._call("energy_balance", C.byref(h), C.byref(r))

@@ -304,6 +304,7 @@ enum FinishKind : int {
     FIN_FCG_BETA,       // <z,r>, <z,q> -> beta, gamma
     FIN_DOT2,           // two plain sums -> out[0], out[1]
     FIN_FCG_SETUP_WARM, // <b,b>, <r,r> of a nonzero initial iterate
+    FIN_DIFF,           // plain sum minus out[1] -> out[0]
 };
 
 // ---- kernel launchers (defined in the .hip files) ----

@@ -886,7 +886,7 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
     if (tid != 0) return;
     // a Krylov reduction that is not finite ends the solve at once (the host reports
     // HS_E_NAN) instead of iterating on to maxit
-    if (kind != FIN_SWEEP && kind != FIN_DOT && kind != FIN_DOT2) {
+    if (kind != FIN_SWEEP && kind != FIN_DOT && kind != FIN_DOT2 && kind != FIN_DIFF) {
         const bool setup = kind == FIN_CG_SETUP || kind == FIN_FCG_SETUP || kind == FIN_FCG_SETUP_WARM;
         if (setup) ctl->nan = 0;
         if (!isfinite(a) || !isfinite(b)) {
@@ -918,6 +918,7 @@ __global__ __launch_bounds__(1024) void k_finish(const double* partials, int n, 
         }
         case FIN_DOT: *out = a; break;
         case FIN_DOT2: out[0] = a; out[1] = b; break;
+        case FIN_DIFF: out[0] = a - out[1]; break;
         case FIN_FCG_SETUP:
             ctl->bb = a; ctl->rr = a; ctl->iters = 0;
             ctl->converged = (a == 0.0) ? 1 : 0; ctl->done = (a == 0.0) ? 1 : 0;
@@ -1041,7 +1042,14 @@ __global__ void k_update_p(Dims d, const uint8_t* cls, const double* pp, double*
     pt[c] = nv;
 }
 
-// Mode C energy balance: sums of cell residuals and of heat input.
+// Mode C energy balance: the sum over all cell rows of a_P T_P - sum_nb a_nb T_nb, the rows
+// of t_row_cfg (the caller subtracts the heat input sum q V).  Both cells of an interior
+// face get the same diffusive part and a_P = sum_nb a_nb, so an interior face with flux F
+// (from its minus to its plus cell) adds F (T_plus - T_minus), and the sum telescopes to
+//   sum_c [ -T_c m_c + sum over c's boundary faces of (a_face (T_c - T_ghost) - F_in T_c) ]
+// with m_c the net outward flux of cell c, F_in a boundary face's flux into c and a_face
+// the row coefficient toward its ghost (t_row_cfg's boundary forms).  Interior cells read
+// T and their six face velocities only; boundary cells also form a_face.
 __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* partials) {
     const Dims& d = a.d;
     const int tid = threadIdx.x + BX * threadIdx.y;
@@ -1049,11 +1057,40 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
     double acc = 0.0;
     if (i <= d.nx && j <= d.ny) {
         const int k0 = 1 + blockIdx.z * kc, k1 = min(d.nz, k0 + kc - 1);
+        const double xp = a.g.xp[i], yp = a.g.yp[j];
+        const double Axy = xp * yp;
+        const double rho = a.rho;
+        const double* T = a.t;
+        const bool edge_ij = i == 1 || i == d.nx || j == 1 || j == d.ny;
+        double wB = a.w[idxW(d, i, j, k0)];
         for (int k = k0; k <= k1; ++k) {
-            const TRowC r = t_row_cfg(a, i, j, k);
-            if (r.cls == CELL_GHOST) continue;
-            acc += r.ap * r.t0 - (r.cf[0] * r.tE + r.cf[1] * r.tW + r.cf[2] * r.tN + r.cf[3] * r.tS + r.cf[4] * r.tB +
-                                  r.cf[5] * r.tT) - r.bs;
+            const long c = idxP(d, i, j, k);
+            const double zp = a.g.zp[k];
+            const double uW = a.u[idxU(d, i, j, k)], uE = a.u[idxU(d, i + 1, j, k)];
+            const double vS = a.v[idxV(d, i, j, k)], vN = a.v[idxV(d, i, j + 1, k)];
+            const double wT = a.w[idxW(d, i, j, k + 1)];
+            const double t0 = T[c];
+            const uint8_t cls = a.cls[c];
+            const double Axz = xp * zp, Ayz = yp * zp;
+            const double FW = rho * Ayz * uW, FE = rho * Ayz * uE;
+            const double FS = rho * Axz * vS, FN = rho * Axz * vN;
+            const double FB = rho * Axy * wB, FT = rho * Axy * wT;
+            wB = wT;
+            if (cls == CELL_GHOST) continue;
+            acc -= t0 * ((FE - FW) + (FN - FS) + (FT - FB));
+            if (edge_ij || k == 1 || k == d.nz) {
+                // a boundary face's coefficient: the cell's own width and conductivity on
+                // both halves (t_row_cfg's out-of-domain neighbour)
+                const double kP = a.kc[idxC(d, i - 1, j - 1, k - 1)];
+                if (i == 1) acc += (Ayz / (0.5 * xp / kP + 0.5 * xp / kP) + posd(FW)) * (t0 - T[c - 1]) - FW * t0;
+                if (i == d.nx) acc += (Ayz / (0.5 * xp / kP + 0.5 * xp / kP) + posd(-FE)) * (t0 - T[c + 1]) + FE * t0;
+                if (j == 1) acc += (Axz / (0.5 * yp / kP + 0.5 * yp / kP) + posd(FS)) * (t0 - T[c - d.sPy]) - FS * t0;
+                if (j == d.ny)
+                    acc += (Axz / (0.5 * yp / kP + 0.5 * yp / kP) + posd(-FN)) * (t0 - T[c + d.sPy]) + FN * t0;
+                if (k == 1) acc += (Axy / (0.5 * zp / kP + 0.5 * zp / kP) + posd(FB)) * (t0 - T[c - d.sPz]) - FB * t0;
+                if (k == d.nz)
+                    acc += (Axy / (0.5 * zp / kP + 0.5 * zp / kP) + posd(-FT)) * (t0 - T[c + d.sPz]) + FT * t0;
+            }
         }
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -1431,9 +1468,6 @@ static int energy_balance_launch(hs_ctx* c) {
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
     const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
-    k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials);
-    HS_CHECK(hipGetLastError());
-    HS_TRY(launch_finish(c, FIN_DOT, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
     if (c->qsum_version != c->q_version) {   // q changed since the last sum (or never summed)
         k_sum_q<<<FLAT_BLOCKS, FLAT, 0, c->stream>>>(d, c->g, c->q, c->cls, c->partials);
         HS_CHECK(hipGetLastError());
@@ -1441,6 +1475,10 @@ static int energy_balance_launch(hs_ctx* c) {
         c->qsum_version = c->q_version;
     }
     HS_CHECK(hipMemcpyAsync(&c->sctl->red[2], c->qsum, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    // red[1] = (sum of a_P T_P - sum_nb a_nb T_nb) - red[2]
+    k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials);
+    HS_CHECK(hipGetLastError());
+    HS_TRY(launch_finish(c, FIN_DIFF, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
     return HS_OK;
 }
 

@@ -1366,6 +1366,16 @@ int or_step(or_ctx* c, hs_stats* st) {
     return HS_OK;
 }
 
+int or_energy_balance(or_ctx* c, double* heat_in, double* residual) {
+    if (!c || !heat_in || !residual) return HS_E_ARG;
+    if (c->o.modeR()) return HS_E_ARG;
+    hs_stats loc{};
+    energy_balance(c->o, &loc);
+    *heat_in = loc.heat_in;
+    *residual = loc.energy_residual;
+    return HS_OK;
+}
+
 int or_run(or_ctx* c, int32_t n, hs_stats* st) {
     for (int32_t it = 0; it < n; ++it) {
         const int e = or_step(c, st ? st + it : nullptr);

@@ -60,6 +60,10 @@ int     or_adjoint_energy(or_ctx* c, const double* dJdT, double* dJdq, double* d
 int     or_energy_solve_rhs(or_ctx* c, int32_t transpose, const double* b, double* x, double tol,
                             int32_t maxit, int32_t mem, hs_solve_info* info);
 int     or_energy_apply(or_ctx* c, int32_t transpose, const double* x, double* y, int32_t mem);
+/* Mode C global energy balance of the current field, as hs_energy_balance reports it:
+ * *heat_in = sum q V, *residual = |sum over cells of the energy row residual| / |heat_in|
+ * (the direct per-cell sum of the step statistics). */
+int     or_energy_balance(or_ctx* c, double* heat_in, double* residual);
 int     or_get_cells(or_ctx* c, int32_t which, double* dst, int32_t mem);
 int     or_put_cells(or_ctx* c, int32_t which, const double* src, int32_t mem);
 

@@ -104,6 +104,41 @@ def test_mode_c_steps(hs_lib, oracle_lib, order):
         assert rel_l2(fg[k], fo[k]) < 1e-9, k
 
 
+@pytest.mark.parametrize("name", ["c2_32x8x8", "c3_32", "c5_64x32x32"])
+def test_energy_balance_matches_oracle(hs_lib, oracle_lib, name):
+    """hs_energy_balance (a telescoped sum: each cell's -T m_c plus its boundary faces'
+    terms) against the oracle's direct sum of every cell row's residual, on one state:
+    after 3 SIMPLE steps (the outflow ghost set by the end of the step), and after the
+    sweeps, correction and energy sweep of a fourth (the ghost one sweep old and the
+    corrected flow not yet divergence-free to round-off).  Channel with inlet, outflow
+    and walls; lid-driven box with inlet; wall-stretched finned channel."""
+    case = {"c2_32x8x8": lambda: configs.c2(scale=8), "c3_32": lambda: configs.c3(32),
+            "c5_64x32x32": lambda: configs.c5(64, 32, 32)}[name]()
+    case.params.order = abi.HS_ORDER_RB
+    g = gpu_ctx(hs_lib, case)
+    g.init()
+    for _ in range(3):
+        g.step()
+    o = oracle_ctx(oracle_lib, case)
+    o.init()
+    for phase in ("end of step", "after the energy sweep"):
+        if phase != "end of step":
+            for comp in range(3):
+                g.momentum(comp)
+            g.pressure_correction()
+            g.correct()
+            g.fluid_energy()
+        o.load_state(g.state())
+        hg, rg = g.energy_balance()
+        ho, ro = o.energy_balance()
+        print(f"[balance] {name} {phase}: heat {hg:.12e} / {ho:.12e}, residual {rg:.12e} / {ro:.12e}", flush=True)
+        assert abs(hg - ho) <= 1e-13 * abs(ho)
+        assert ro > 1e-6
+        assert abs(rg - ro) <= 1e-10 * max(ro, 1.0)
+    g.close()
+    o.close()
+
+
 def test_mode_c_phases_bitexact(hs_lib, oracle_lib):
     case = configs.c2(scale=8)
     case.params.order = abi.HS_ORDER_RB
