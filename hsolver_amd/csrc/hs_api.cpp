@@ -35,7 +35,7 @@ double* array_ptr(hs_ctx* c, int which) {
     switch (which) {
         case HS_A_U: return c->u; case HS_A_V: return c->v; case HS_A_W: return c->w;
         case HS_A_UP: return c->up; case HS_A_VP: return c->vp; case HS_A_WP: return c->wp;
-        case HS_A_P: return c->p; case HS_A_PTEMP: return c->ptemp; case HS_A_PPRIME: return c->pprime;
+        case HS_A_P: return c->p; case HS_A_PTEMP: return c->ptemp; case HS_A_PPRIME: return pprime_now(c);
         case HS_A_TF: return c->tf; case HS_A_TS: return c->ts; case HS_A_Q: return c->q;
         case HS_A_K: return c->k; case HS_A_B: return c->b;
     }
@@ -215,7 +215,8 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
     else e = cg_solve(c, c->b, c->kx, &loc, true);
     c->kx_hist = e == HS_OK ? std::min(c->kx_hist + 1, 3) : 0;
     if (e != HS_OK) return e;
-    HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
+    // p' is kx until the correction consumes it (the pprime buffer is not copied into)
+    c->pprime_in_kx = true;
     if (st) {
         st->p_iters = loc.p_iters; st->p_converged = loc.p_converged; st->p_relres = loc.p_relres;
         st->p_true_relres = loc.p_true_relres; st->mass_imbalance = loc.mass_imbalance;
@@ -237,7 +238,14 @@ int do_correct(hs_ctx* c) {
 // fluid cells zeroed, the configured Krylov method into kx, the true residual and ||b||
 // recomputed.  The reductions use c->partials and c->sctl as they are at the call.
 int pressure_solve_staged(hs_ctx* c, hs_stats* st) {
-    c->kx_hist = 0;   // kx is about to hold another system's solution
+    // kx is about to hold another system's solution: a p' still held there moves to the
+    // pprime buffer first
+    if (c->pprime_in_kx) {
+        HS_TRY(launch_copy(c, c->pprime, c->kx, c->d.nP));
+        c->pprime_in_kx = false;
+        c->pprime_dirty = true;
+    }
+    c->kx_hist = 0;
     HS_TRY(mask_fluid(c, c->kb));
     hs_stats loc{};
     const int e = (c->prm.precond != HS_PRECOND_NONE) ? fcg_solve(c, c->kb, c->kx, &loc)
@@ -759,6 +767,10 @@ int hs_put_array(hs_ctx* c, int32_t which, const double* host, int64_t n) {
     if (!c || !host) return HS_E_ARG;
     const int64_t s = array_len(c, which);
     if (s < 0 || n != s) return HS_E_ARG;
+    if (which == HS_A_PPRIME) {   // the written values are p' from now on
+        c->pprime_in_kx = false;
+        c->pprime_dirty = true;
+    }
     HS_CHECK(hipMemcpyAsync(array_ptr(c, which), host, s * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (which == HS_A_Q) ++c->q_version;
     HS_CHECK(hipStreamSynchronize(c->stream));

@@ -251,6 +251,10 @@ struct hs_ctx {
     // after q changed (q_version counts the writes of q)
     double* qsum = nullptr;
     long q_version = 0, qsum_version = -1;
+    // p' between the solve and the correction: the solve's output kx stands for the
+    // pprime array (no copy), and the pprime buffer itself stays zero unless written
+    // through hs_put_array (pprime_dirty)
+    bool pprime_in_kx = false, pprime_dirty = false;
     // p' operator: Mode R per-cell row coefficients (P), Mode C per-face (U/V/W)
     double *ax = nullptr, *ay = nullptr, *az = nullptr;
     // Krylov work (P layout)
@@ -306,6 +310,9 @@ enum FinishKind : int {
     FIN_FCG_SETUP_WARM, // <b,b>, <r,r> of a nonzero initial iterate
     FIN_DIFF,           // plain sum minus out[1] -> out[0]
 };
+
+// the array that holds p' now (the solve's output, or the pprime buffer)
+inline double* pprime_now(const hs_ctx* c) { return c->pprime_in_kx ? c->kx : c->pprime; }
 
 // ---- kernel launchers (defined in the .hip files) ----
 int launch_finish(hs_ctx* c, int kind, int n, double* out);

@@ -1450,16 +1450,24 @@ int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, dou
 int correct_velocity(hs_ctx* c) {
     const Dims& d = c->d;
     k_correct<<<dim3((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1), 64, 0, c->stream>>>(
-        d, c->g, c->cls, c->pprime, c->u, c->v, c->w, c->up, c->vp, c->wp, c->prm.mode == HS_MODE_REF);
+        d, c->g, c->cls, pprime_now(c), c->u, c->v, c->w, c->up, c->vp, c->wp, c->prm.mode == HS_MODE_REF);
     HS_CHECK(hipGetLastError());
     return HS_OK;
 }
 
+// p += relax_p p', then p' = 0 (fluid_solver.h:1580-1597): when p' is the solve's output
+// in kx the pprime buffer already holds zeros, otherwise it is cleared
 int update_pressure(hs_ctx* c) {
     const Dims& d = c->d;
-    k_update_p<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, c->pprime, c->p, c->ptemp,
+    k_update_p<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, pprime_now(c), c->p, c->ptemp,
                                                                          c->prm.relax_p);
     HS_CHECK(hipGetLastError());
+    if (c->pprime_in_kx && !c->pprime_dirty) {
+        c->pprime_in_kx = false;
+        return HS_OK;
+    }
+    c->pprime_in_kx = false;
+    c->pprime_dirty = false;
     return launch_fill(c, c->pprime, d.nP, 0.0);
 }
 

@@ -159,6 +159,37 @@ def test_mode_c_phases_bitexact(hs_lib, oracle_lib):
         assert_same(g, o, what=f"iter {it} set_for_next")
 
 
+@pytest.mark.parametrize("precond", [abi.HS_PRECOND_NONE, abi.HS_PRECOND_GMG])
+def test_pprime_array_between_solve_and_correction(hs_lib, oracle_lib, precond):
+    """The p' array between the pressure solve and the correction reads back as the solve's
+    solution (the oracle's to 1e-9, both solved to 1e-12); a pressure solve of another right-hand side in
+    between changes neither it nor what the correction does; after the correction it
+    reads back as zero.  (A p' written through hs_put_array before the correction is the
+    one the correction uses: test_mode_c_phases_bitexact.)"""
+    case = configs.c2(scale=8)
+    case.params.order = abi.HS_ORDER_RB
+    case.params.p_tol = 1e-12           # two solvers' solutions agree far below the bar
+    p = abi.Params.from_buffer_copy(case.params)
+    p.precond = precond
+    g1, g2, o = gpu_ctx(hs_lib, case, p), gpu_ctx(hs_lib, case, p), oracle_ctx(oracle_lib, case)
+    for c in (g1, g2, o):
+        c.init()
+        for comp in range(3):
+            c.momentum(comp)
+        c.pressure_correction()
+    x1 = g1.get("pprime")
+    assert rel_l2(x1, o.get("pprime")) < 1e-9
+    b = np.random.default_rng(11).standard_normal(x1.shape)
+    xs, _ = g2.pressure_solve(b)                 # another system through the solver's buffers
+    assert rel_l2(xs, x1) > 1e-3
+    assert np.array_equal(g2.get("pprime"), x1)
+    g1.correct(); g2.correct()
+    assert_same(g1, g2, what="correction after another solve")
+    assert not np.any(g1.get("pprime")) and not np.any(g2.get("pprime"))
+    for c in (g1, g2, o):
+        c.close()
+
+
 def test_pressure_operator_and_cg(hs_lib, oracle_lib):
     case = configs.c2(scale=4)          # 64x16x16
     case.params.order = abi.HS_ORDER_RB
