@@ -1048,8 +1048,9 @@ __global__ void k_update_p(Dims d, const uint8_t* cls, const double* pp, double*
 // (from its minus to its plus cell) adds F (T_plus - T_minus), and the sum telescopes to
 //   sum_c [ -T_c m_c + sum over c's boundary faces of (a_face (T_c - T_ghost) - F_in T_c) ]
 // with m_c the net outward flux of cell c, F_in a boundary face's flux into c and a_face
-// the row coefficient toward its ghost (t_row_cfg's boundary forms).  Interior cells read
-// T and their six face velocities only; boundary cells also form a_face.
+// the row coefficient toward its ghost (t_row_cfg's boundary forms).  k_energy_balance
+// adds the cells' -T_c m_c (T, class and six face velocities per cell),
+// k_energy_balance_faces the boundary faces' terms.
 __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* partials) {
     const Dims& d = a.d;
     const int tid = threadIdx.x + BX * threadIdx.y;
@@ -1060,8 +1061,6 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
         const double xp = a.g.xp[i], yp = a.g.yp[j];
         const double Axy = xp * yp;
         const double rho = a.rho;
-        const double* T = a.t;
-        const bool edge_ij = i == 1 || i == d.nx || j == 1 || j == d.ny;
         double wB = a.w[idxW(d, i, j, k0)];
         for (int k = k0; k <= k1; ++k) {
             const long c = idxP(d, i, j, k);
@@ -1069,7 +1068,7 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
             const double uW = a.u[idxU(d, i, j, k)], uE = a.u[idxU(d, i + 1, j, k)];
             const double vS = a.v[idxV(d, i, j, k)], vN = a.v[idxV(d, i, j + 1, k)];
             const double wT = a.w[idxW(d, i, j, k + 1)];
-            const double t0 = T[c];
+            const double t0 = a.t[c];
             const uint8_t cls = a.cls[c];
             const double Axz = xp * zp, Ayz = yp * zp;
             const double FW = rho * Ayz * uW, FE = rho * Ayz * uE;
@@ -1078,23 +1077,76 @@ __global__ __launch_bounds__(BT) void k_energy_balance(TArgs a, int kc, double* 
             wB = wT;
             if (cls == CELL_GHOST) continue;
             acc -= t0 * ((FE - FW) + (FN - FS) + (FT - FB));
-            if (edge_ij || k == 1 || k == d.nz) {
-                // a boundary face's coefficient: the cell's own width and conductivity on
-                // both halves (t_row_cfg's out-of-domain neighbour)
-                const double kP = a.kc[idxC(d, i - 1, j - 1, k - 1)];
-                if (i == 1) acc += (Ayz / (0.5 * xp / kP + 0.5 * xp / kP) + posd(FW)) * (t0 - T[c - 1]) - FW * t0;
-                if (i == d.nx) acc += (Ayz / (0.5 * xp / kP + 0.5 * xp / kP) + posd(-FE)) * (t0 - T[c + 1]) + FE * t0;
-                if (j == 1) acc += (Axz / (0.5 * yp / kP + 0.5 * yp / kP) + posd(FS)) * (t0 - T[c - d.sPy]) - FS * t0;
-                if (j == d.ny)
-                    acc += (Axz / (0.5 * yp / kP + 0.5 * yp / kP) + posd(-FN)) * (t0 - T[c + d.sPy]) + FN * t0;
-                if (k == 1) acc += (Axy / (0.5 * zp / kP + 0.5 * zp / kP) + posd(FB)) * (t0 - T[c - d.sPz]) - FB * t0;
-                if (k == d.nz)
-                    acc += (Axy / (0.5 * zp / kP + 0.5 * zp / kP) + posd(-FT)) * (t0 - T[c + d.sPz]) + FT * t0;
-            }
         }
     }
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     put_partial<BT>(acc, tid, bid, partials);
+}
+
+// The boundary faces' terms a_face (T_c - T_ghost) - F_in T_c, one face per thread: the
+// x = min / max faces (ny nz each), then y (nx nz each), then z (nx ny each).  a_face takes
+// the cell's own width and conductivity on both halves (t_row_cfg's out-of-domain
+// neighbour).
+__global__ __launch_bounds__(FLAT) void k_energy_balance_faces(TArgs a, double* partials) {
+    const Dims& d = a.d;
+    const long nyz = (long)d.ny * d.nz, nxz = (long)d.nx * d.nz, nxy = (long)d.nx * d.ny;
+    const long total = 2 * (nyz + nxz + nxy);
+    const double rho = a.rho;
+    const double* T = a.t;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)FLAT + threadIdx.x; t < total; t += (long)gridDim.x * FLAT) {
+        long f = t;
+        int axis, hi;
+        long i, j, k;
+        if (f < 2 * nyz) {
+            axis = 0; hi = f >= nyz; f -= hi * nyz;
+            j = 1 + f % d.ny; k = 1 + f / d.ny; i = hi ? d.nx : 1;
+        } else if ((f -= 2 * nyz) < 2 * nxz) {
+            axis = 1; hi = f >= nxz; f -= hi * nxz;
+            i = 1 + f % d.nx; k = 1 + f / d.nx; j = hi ? d.ny : 1;
+        } else {
+            f -= 2 * nxz;
+            axis = 2; hi = f >= nxy; f -= hi * nxy;
+            i = 1 + f % d.nx; j = 1 + f / d.nx; k = hi ? d.nz : 1;
+        }
+        const long c = idxP(d, i, j, k);
+        if (a.cls[c] == CELL_GHOST) continue;
+        const double xp = a.g.xp[i], yp = a.g.yp[j], zp = a.g.zp[k];
+        const double kP = a.kc[idxC(d, i - 1, j - 1, k - 1)];
+        const double t0 = T[c];
+        const double Axy = xp * yp, Axz = xp * zp, Ayz = yp * zp;
+        double term;
+        if (axis == 0) {
+            const double D = Ayz / (0.5 * xp / kP + 0.5 * xp / kP);
+            if (!hi) {
+                const double FW = rho * Ayz * a.u[idxU(d, i, j, k)];
+                term = (D + posd(FW)) * (t0 - T[c - 1]) - FW * t0;
+            } else {
+                const double FE = rho * Ayz * a.u[idxU(d, i + 1, j, k)];
+                term = (D + posd(-FE)) * (t0 - T[c + 1]) + FE * t0;
+            }
+        } else if (axis == 1) {
+            const double D = Axz / (0.5 * yp / kP + 0.5 * yp / kP);
+            if (!hi) {
+                const double FS = rho * Axz * a.v[idxV(d, i, j, k)];
+                term = (D + posd(FS)) * (t0 - T[c - d.sPy]) - FS * t0;
+            } else {
+                const double FN = rho * Axz * a.v[idxV(d, i, j + 1, k)];
+                term = (D + posd(-FN)) * (t0 - T[c + d.sPy]) + FN * t0;
+            }
+        } else {
+            const double D = Axy / (0.5 * zp / kP + 0.5 * zp / kP);
+            if (!hi) {
+                const double FB = rho * Axy * a.w[idxW(d, i, j, k)];
+                term = (D + posd(FB)) * (t0 - T[c - d.sPz]) - FB * t0;
+            } else {
+                const double FT = rho * Axy * a.w[idxW(d, i, j, k + 1)];
+                term = (D + posd(-FT)) * (t0 - T[c + d.sPz]) + FT * t0;
+            }
+        }
+        acc += term;
+    }
+    put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
 __global__ void k_sum_q(Dims d, Geo g, const double* q, const uint8_t* cls, double* partials) {
@@ -1483,10 +1535,17 @@ static int energy_balance_launch(hs_ctx* c) {
         c->qsum_version = c->q_version;
     }
     HS_CHECK(hipMemcpyAsync(&c->sctl->red[2], c->qsum, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    // red[1] = (sum of a_P T_P - sum_nb a_nb T_nb) - red[2]
-    k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(t_args(c, false), kc, c->partials);
+    // red[1] = (sum of a_P T_P - sum_nb a_nb T_nb) - red[2]: the cells' partials, then the
+    // boundary faces' behind them, finished together
+    const int nbc = (int)(gr.x * gr.y * gr.z);
+    const long faces = 2 * ((long)d.ny * d.nz + (long)d.nx * d.nz + (long)d.nx * d.ny);
+    const int nbf = (int)std::min<long>(1024, (faces + FLAT - 1) / FLAT);
+    const TArgs ta = t_args(c, false);
+    k_energy_balance<<<gr, dim3(BX, BY), 0, c->stream>>>(ta, kc, c->partials);
     HS_CHECK(hipGetLastError());
-    HS_TRY(launch_finish(c, FIN_DIFF, (int)(gr.x * gr.y * gr.z), &c->sctl->red[1]));
+    k_energy_balance_faces<<<nbf, FLAT, 0, c->stream>>>(ta, c->partials + nbc);
+    HS_CHECK(hipGetLastError());
+    HS_TRY(launch_finish(c, FIN_DIFF, nbc + nbf, &c->sctl->red[1]));
     return HS_OK;
 }
 
