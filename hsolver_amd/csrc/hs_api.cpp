@@ -226,8 +226,7 @@ int do_pressure(hs_ctx* c, hs_stats* st) {
 }
 
 int do_correct(hs_ctx* c) {
-    HS_TRY(correct_velocity(c));
-    HS_TRY(update_pressure(c));
+    HS_TRY(correct_and_update(c));
     if (!modeR(c)) HS_TRY(bc_cfg_outflow(c, 0));
     return HS_OK;
 }

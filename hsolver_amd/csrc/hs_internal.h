@@ -345,8 +345,7 @@ int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, dou
 // queues ||b - Ax||^2 and ||b||^2 into sctl->red[0], red[1] (no synchronisation)
 int launch_resid_norms(hs_ctx* c, const double* b, const double* x);
 void resid_from_red(const SolveCtl* h, hs_stats* st);
-int correct_velocity(hs_ctx* c);
-int update_pressure(hs_ctx* c);
+int correct_and_update(hs_ctx* c);   // velocity correction, p += relax_p p', p' = 0
 int energy_sweep(hs_ctx* c);
 int solid_sweep(hs_ctx* c);
 int exchange_kernel(hs_ctx* c);

@@ -952,9 +952,12 @@ __global__ __launch_bounds__(BT) void k_spmv(OpArgs o, const double* x, double* 
     const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
     if (i > d.nx || j > d.ny) return;
     const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
-    for (int k = k0; k <= k1; ++k) y[idxP(d, i, j, k)] = 0.0;
+    // every cell of the column is written once: the row's value, or zero off the fluid
     spmv_column(
-        o, [x](long q) { return x[q]; }, i, j, k0, k1, [&](long c, double v, double) { y[c] = v; }, [](long, double) {});
+        o, [x](long q) { return x[q]; }, i, j, k0, k1, [&](long c, double v, double) { y[c] = v; },
+        [&](long c, double) {
+            if (o.cls[c] != CELL_FLUID) y[c] = 0.0;
+        });
 }
 
 // ||b - A x||^2 and ||b||^2 in one pass over the fluid rows (block partials, two per
@@ -993,9 +996,12 @@ __global__ void k_axpby(long n, double a, const double* x, double b, const doubl
 
 // ------------------------------------------------------- corrections ---
 // velocity correction fluid_solver.h:1459-1571 (Mode R: faces whose plus-side cell is
-// fluid; Mode C: faces between two fluid cells)
+// fluid; Mode C: faces between two fluid cells), and the pressure update
+// fluid_solver.h:1574-1598 of the thread's own cell (p += relax p', p_temp = p on fluid
+// cells; p' is only read, so the order of the two does not matter)
 __global__ void k_correct(Dims d, Geo g, const uint8_t* cls, const double* pp, double* u, double* v, double* w,
-                          const double* up, const double* vp, const double* wp, int modeR) {
+                          const double* up, const double* vp, const double* wp, int modeR, double* p, double* pt,
+                          double relax) {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
     auto fl = [&](long ii, long jj, long kk) { return cls[idxP(d, ii, jj, kk)] == CELL_FLUID; };
     const int ex = modeR ? 0 : 1;
@@ -1029,17 +1035,12 @@ __global__ void k_correct(Dims d, Geo g, const uint8_t* cls, const double* pp, d
             w[idxW(d, i, j, k)] = w[idxW(d, i, j, k)] + corr;
         }
     }
-}
-// pressure update fluid_solver.h:1574-1598: p += relax p', p_temp = p on fluid cells.
-__global__ void k_update_p(Dims d, const uint8_t* cls, const double* pp, double* p, double* pt, double relax) {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
-    if (i > d.nx) return;
-    const long c = idxP(d, i, j, k);
-    if (cls[c] != CELL_FLUID) return;
-    const long cc = idxC(d, i - 1, j - 1, k - 1);
-    const double nv = p[cc] + relax * pp[c];
-    p[cc] = nv;
-    pt[c] = nv;
+    if (i <= d.nx && j <= d.ny && k <= d.nz && fl(i, j, k)) {
+        const long c = idxP(d, i, j, k), cc = idxC(d, i - 1, j - 1, k - 1);
+        const double nv = p[cc] + relax * pp[c];
+        p[cc] = nv;
+        pt[c] = nv;
+    }
 }
 
 // Mode C energy balance: the sum over all cell rows of a_P T_P - sum_nb a_nb T_nb, the rows
@@ -1499,20 +1500,14 @@ int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, dou
     return HS_OK;
 }
 
-int correct_velocity(hs_ctx* c) {
+// velocity correction and pressure update (one pass), then p' = 0
+// (fluid_solver.h:1459-1597): when p' is the solve's output in kx the pprime buffer already
+// holds zeros, otherwise it is cleared
+int correct_and_update(hs_ctx* c) {
     const Dims& d = c->d;
     k_correct<<<dim3((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1), 64, 0, c->stream>>>(
-        d, c->g, c->cls, pprime_now(c), c->u, c->v, c->w, c->up, c->vp, c->wp, c->prm.mode == HS_MODE_REF);
-    HS_CHECK(hipGetLastError());
-    return HS_OK;
-}
-
-// p += relax_p p', then p' = 0 (fluid_solver.h:1580-1597): when p' is the solve's output
-// in kx the pprime buffer already holds zeros, otherwise it is cleared
-int update_pressure(hs_ctx* c) {
-    const Dims& d = c->d;
-    k_update_p<<<dim3((d.nx + 63) / 64, d.ny, d.nz), 64, 0, c->stream>>>(d, c->cls, pprime_now(c), c->p, c->ptemp,
-                                                                         c->prm.relax_p);
+        d, c->g, c->cls, pprime_now(c), c->u, c->v, c->w, c->up, c->vp, c->wp, c->prm.mode == HS_MODE_REF, c->p,
+        c->ptemp, c->prm.relax_p);
     HS_CHECK(hipGetLastError());
     if (c->pprime_in_kx && !c->pprime_dirty) {
         c->pprime_in_kx = false;
