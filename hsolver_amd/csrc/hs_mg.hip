@@ -454,30 +454,50 @@ __global__ __launch_bounds__(TX * TY) void k_mg_restrict_s(Lv<T> F, Lv<T> C, int
 // Dirichlet value of walls and solids.  With summed (constant) restriction the
 // transfer orders satisfy m_P + m_R > 2m for this second-order operator.
 template <typename T>
-__device__ __forceinline__ T prolong_value(const Lv<T>& C, int i, int j, int k) {
+__device__ __forceinline__ T prolong_value(const Lv<T>& C, int i, int j, int k, const T* __restrict__ x) {
     const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
     const int di = (i & 1) ? -1 : 1, dj = (j & 1) ? -1 : 1, dk = (k & 1) ? -1 : 1;
     const long c0 = I + C.sy * J + C.sz * K;
     const long ox = di, oy = dj * C.sy, oz = dk * C.sz;
-    const T* x = C.x;
     const T a = T(0.75), b = T(0.25);
     const T zlo = a * (a * x[c0] + b * x[c0 + ox]) + b * (a * x[c0 + oy] + b * x[c0 + ox + oy]);
     const T zhi = a * (a * x[c0 + oz] + b * x[c0 + ox + oz]) + b * (a * x[c0 + oy + oz] + b * x[c0 + ox + oy + oz]);
     return a * zlo + b * zhi;
 }
 
-// x_f += P x_c on active fine cells (64 x 4 fine columns per workgroup, z chunks)
+// x_f += P x_c on active fine cells (64 x 4 fine columns per workgroup, z chunks).  The
+// planes are independent: four at a time, with the mask, the fine iterate and the eight
+// coarse values of all four loaded before any store (whatever the mask says; a group
+// past the chunk's last plane loads that plane again and does not store it), so four
+// planes cost one memory round trip.
 template <typename T>
 __global__ __launch_bounds__(BT) void k_mg_prolong(Lv<T> F, Lv<T> C, int kc, int gx, int gy, int gz,
                                                   const SolveCtl* ctl) {
+    constexpr int U = 4;
     if (ctl->done) return;
     const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
     const int i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
     if (i > F.nx || j > F.ny) return;
     const int k0 = 1 + tl.bz * kc, k1 = min(F.nz, k0 + kc - 1);
-    for (int k = k0; k <= k1; ++k) {
-        const long f = i + F.sy * j + F.sz * k;
-        if (F.act[f]) F.x[f] += prolong_value(C, i, j, k);
+    T* __restrict__ xf = F.x;
+    const T* __restrict__ xc = C.x;
+    const uint8_t* __restrict__ act = F.act;
+    const long col = i + F.sy * j;
+    for (int k = k0; k <= k1; k += U) {
+        int on[U];
+        T xv[U], pv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u, k1);
+            const long f = col + F.sz * kk;
+            on[u] = act[f];
+            xv[u] = xf[f];
+            pv[u] = prolong_value(C, i, j, kk, xc);
+        }
+        hsd::issued_together(on[0], on[1], on[2], on[3], xv[0], xv[1], xv[2], xv[3], pv[0], pv[1], pv[2], pv[3]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u] && k + u <= k1) xf[col + F.sz * (k + u)] = xv[u] + pv[u];
     }
 }
 
@@ -600,7 +620,7 @@ __device__ void tail_prolong(const Lv<T>& F, const Lv<T>& C) {
         const int i = 1 + (int)(t % F.nx), j = 1 + (int)((t / F.nx) % F.ny), k = 1 + (int)(t / ((long)F.nx * F.ny));
         const long f = i + F.sy * j + F.sz * k;
         if (!F.act[f]) continue;
-        F.x[f] += prolong_value(C, i, j, k);
+        F.x[f] += prolong_value(C, i, j, k, C.x);
     }
     __syncthreads();
 }
