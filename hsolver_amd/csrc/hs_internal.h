@@ -338,8 +338,8 @@ int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
 // are formed on the device before each readback of the solve's control block and read back
 // with it, so a converged solve returns them without another synchronisation
 int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid = false);
-int cg_spmv_dot(hs_ctx* c, const double* p, double* q);
-int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* pn, double* q);
+int cg_spmv_dot(hs_ctx* c, const void* p, bool f32, double* q);
+int fcg_spmv_dot(hs_ctx* c, const void* z, bool f32, const void* p, void* pn, double* q);
 int mask_fluid(hs_ctx* c, double* x);
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);
 // queues ||b - Ax||^2 and ||b||^2 into sctl->red[0], red[1] (no synchronisation)

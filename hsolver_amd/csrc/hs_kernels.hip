@@ -750,8 +750,10 @@ __device__ __forceinline__ void cg_update_record(SolveCtl* ctl, const CgIn& ci, 
     ctl->beta = s.v;
 }
 
-// s = A p (z-marching columns) with block partials of <s,p> (k_finish: alpha).
-__global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, double* s, int kc, int gx, int gy, int gz,
+// s = A p (z-marching columns) with block partials of <s,p> (k_finish: alpha); p is float
+// in the mixed-precision flexible CG, the product and the sum FP64.
+template <typename P>
+__global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const P* p, double* s, int kc, int gx, int gy, int gz,
                                                 double* partials, SolveCtl* ctl) {
     if (ctl->done) return;
     const Dims& d = o.d;
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, doubl
     if (i <= d.nx && j <= d.ny) {
         const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
         spmv_column(
-            o, [p](long q) { return p[q]; }, i, j, k0, k1,
+            o, [p](long q) { return (double)p[q]; }, i, j, k0, k1,
             [&](long c, double y, double pc) {
                 s[c] = y;
                 acc += y * pc;
@@ -776,12 +778,15 @@ __global__ __launch_bounds__(BT) void k_cg_spmv(OpArgs o, const double* p, doubl
 // formed at every point the stencil reads (the same expression as a separate update pass,
 // so the same values), written to pn for the column's cells, and q = A p_new with block
 // partials of <p_new, q>.  pn must not alias p (neighbouring columns still read p_old).
+// The directions are stored as T: the mixed-precision solve keeps them in float, formed
+// in FP64 and rounded once, and the apply and <p_new, q> use the rounded value, so q is
+// the FP64 product of the stored direction.
 // With fold (plain CG, z = r), beta is not read from ctl: every block takes the stop rule
 // and beta of CG update fold_it itself from that update's nfold block partials of <r,r>
 // (cg_update_step), block 0 records them, and a finished solve returns here.
 template <typename T>
-__global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__ z, const double* __restrict__ p,
-                                                 double* __restrict__ pn, double* __restrict__ s, int kc, int gx, int gy,
+__global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__ z, const T* __restrict__ p,
+                                                 T* __restrict__ pn, double* __restrict__ s, int kc, int gx, int gy,
                                                  int gz, double* partials, SolveCtl* ctl, const double* fold, int nfold,
                                                  int fold_it) {
     const int tid = threadIdx.x + BX * threadIdx.y;
@@ -808,12 +813,12 @@ __global__ __launch_bounds__(BT) void k_fcg_spmv(OpArgs o, const T* __restrict__
     if (i <= d.nx && j <= d.ny) {
         const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
         spmv_column(
-            o, [=](long q) { return (double)z[q] + beta * p[q]; }, i, j, k0, k1,
+            o, [=](long q) { return (double)(T)((double)z[q] + beta * (double)p[q]); }, i, j, k0, k1,
             [&](long c, double y, double pc) {
                 s[c] = y;
                 acc += y * pc;
             },
-            [=](long c, double pc) { pn[c] = pc; });
+            [=](long c, double pc) { pn[c] = (T)pc; });
     }
     put_partial<BT>(acc, tid, blockIdx.x, partials);
 }
@@ -1442,7 +1447,8 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resi
         for (; in_batch < batch && launched < c->prm.p_maxit; ++in_batch, ++launched) {
             const int slot = prof_begin(c, HS_K_SPMV);
             if (launched == 0) {
-                k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, p, c->ks, kc, gr.x, gr.y, gr.z, sp, c->sctl);
+                k_cg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(o, p, c->ks, kc, gr.x, gr.y, gr.z, sp,
+                                                                             c->sctl);
             } else {
                 // the batch's first apply reads beta from the control block (k_finish)
                 k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(
@@ -1567,32 +1573,41 @@ int mask_fluid(hs_ctx* c, double* x) {
     return HS_OK;
 }
 
-// pn = z + beta p, q = A pn with <pn,q> and alpha (flexible CG, iterations after the first)
-int fcg_spmv_dot(hs_ctx* c, const void* z, bool zf32, const double* p, double* pn, double* q) {
+// pn = z + beta p, q = A pn with <pn,q> and alpha (flexible CG, iterations after the first);
+// f32: z, p and pn are float
+int fcg_spmv_dot(hs_ctx* c, const void* z, bool f32, const void* p, void* pn, double* q) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
     const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
     const int slot = prof_begin(c, HS_K_SPMV);
-    if (zf32)
-        k_fcg_spmv<float><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const float*)z, p, pn, q, kc, gr.x,
-                                                                      gr.y, gr.z, c->partials, c->sctl, nullptr, 0, 0);
+    if (f32)
+        k_fcg_spmv<float><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(
+            op_args(c), (const float*)z, (const float*)p, (float*)pn, q, kc, gr.x, gr.y, gr.z, c->partials, c->sctl,
+            nullptr, 0, 0);
     else
-        k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const double*)z, p, pn, q, kc, gr.x,
-                                                                       gr.y, gr.z, c->partials, c->sctl, nullptr, 0, 0);
+        k_fcg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(
+            op_args(c), (const double*)z, (const double*)p, (double*)pn, q, kc, gr.x, gr.y, gr.z, c->partials, c->sctl,
+            nullptr, 0, 0);
     HS_CHECK(hipGetLastError());
     HS_TRY(prof_end(c, slot));
     return launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr);
 }
 
-// q = A p with <p,q> and alpha = gamma / <p,q> in the last block (shared by CG and FCG).
-int cg_spmv_dot(hs_ctx* c, const double* p, double* q) {
+// q = A p with <p,q> and alpha = gamma / <p,q> in the last block (flexible CG's first
+// iteration; f32: p is float).
+int cg_spmv_dot(hs_ctx* c, const void* p, bool f32, double* q) {
     const Dims& d = c->d;
     const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
     const int kc = kchunk(d.nz, tiles);
     const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
     const int slot = prof_begin(c, HS_K_SPMV);
-    k_cg_spmv<<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), p, q, kc, gr.x, gr.y, gr.z, c->partials, c->sctl);
+    if (f32)
+        k_cg_spmv<float><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const float*)p, q, kc, gr.x, gr.y,
+                                                                     gr.z, c->partials, c->sctl);
+    else
+        k_cg_spmv<double><<<grid1(gr), dim3(BX, BY), 0, c->stream>>>(op_args(c), (const double*)p, q, kc, gr.x, gr.y,
+                                                                      gr.z, c->partials, c->sctl);
     HS_CHECK(hipGetLastError());
     HS_TRY(prof_end(c, slot));
     return launch_finish(c, FIN_CG_ALPHA, (int)(gr.x * gr.y * gr.z), nullptr);

@@ -706,16 +706,16 @@ __global__ __launch_bounds__(FLAT) void k_extrapolate(long n, int ord, const dou
     }
 }
 
-// p = z (the level-0 iterate); partials of <r, z>
+// p = z (the level-0 iterate; the direction is stored as T); partials of <r, z>
 template <typename T>
-__global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const T* z, double* p, double* partials,
+__global__ __launch_bounds__(FLAT) void k_fcg_start(long n, const double* r, const T* z, T* p, double* partials,
                                                     const SolveCtl* ctl) {
     if (ctl->done) return;
     double acc = 0.0;
     GRID_STRIDE(t, n) {
-        const double zv = (double)z[t];
-        p[t] = zv;
-        acc += r[t] * zv;
+        const T zt = z[t];
+        p[t] = zt;
+        acc += r[t] * (double)zt;
     }
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
@@ -727,20 +727,21 @@ template <> struct Pair<float> { using V = float2; };
 template <> struct Pair<double> { using V = double2; };
 
 // x += alpha p, r -= alpha q; level-0 right-hand side = (T) r, iterate = 0 (unless the
-// first sweep reads it as zero); partials of <r,r>
+// first sweep reads it as zero); partials of <r,r>.  The direction p is stored as T.
 template <typename T>
-__global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const double* p, const double* q,
+__global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* r, const T* p, const double* q,
                                                      T* b0, T* x0, int zero_x0, double* partials, const SolveCtl* ctl) {
     using V = typename Pair<T>::V;
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double acc = 0.0;
     GRID_STRIDE(t, n >> 1) {
-        const double2 xv = ((const double2*)x)[t], pv = ((const double2*)p)[t];
+        const double2 xv = ((const double2*)x)[t];
+        const V pv = ((const V*)p)[t];
         const double2 rv = ((const double2*)r)[t], qv = ((const double2*)q)[t];
         issued_together(xv.x, xv.y, pv.x, pv.y, rv.x, rv.y, qv.x, qv.y);
         const double r0 = rv.x - alpha * qv.x, r1 = rv.y - alpha * qv.y;
-        ((double2*)x)[t] = double2{xv.x + alpha * pv.x, xv.y + alpha * pv.y};
+        ((double2*)x)[t] = double2{xv.x + alpha * (double)pv.x, xv.y + alpha * (double)pv.y};
         ((double2*)r)[t] = double2{r0, r1};
         acc += r0 * r0 + r1 * r1;
         ((V*)b0)[t] = V{(T)r0, (T)r1};
@@ -748,7 +749,7 @@ __global__ __launch_bounds__(FLAT) void k_fcg_update(long n, double* x, double* 
     }
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
         const long t = n - 1;
-        x[t] += alpha * p[t];
+        x[t] += alpha * (double)p[t];
         const double rv = r[t] - alpha * q[t];
         r[t] = rv;
         acc += rv * rv;
@@ -972,7 +973,9 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
     h->maxit = c->prm.p_maxit;
     HS_CHECK(hipMemcpyAsync(&c->sctl->eps, &h->eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(&c->sctl->maxit, &h->maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    double *r = c->kr, *p = c->kp, *p2 = c->kp2, *q = c->ks;
+    // directions in T (float in the mixed-precision solve), in the FP64 buffers' storage
+    double *r = c->kr, *q = c->ks;
+    T *p = (T*)c->kp, *p2 = (T*)c->kp2;
     T* b0 = (T*)c->mg[0]->b;
     // the level-0 iterate (the V-cycle's output z) moves between two buffers with the
     // smoother's sweeps: read its current pointer at every use
@@ -1016,7 +1019,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
             // the first direction is p = z (k_fcg_start); later ones are formed by the
             // fused update + apply into the second buffer, and the buffers swap
             if (launched == 0) {
-                HS_TRY(cg_spmv_dot(c, p, q));
+                HS_TRY(cg_spmv_dot(c, p, sizeof(T) == sizeof(float), q));
             } else {
                 HS_TRY(fcg_spmv_dot(c, z(), sizeof(T) == sizeof(float), p, p2, q));
                 std::swap(p, p2);
