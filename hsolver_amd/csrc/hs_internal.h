@@ -339,6 +339,10 @@ int dot(hs_ctx* c, const double* a, const double* b, double* out_host);
 // with it, so a converged solve returns them without another synchronisation
 int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid = false);
 int cg_spmv_dot(hs_ctx* c, const void* p, bool f32, double* q);
+// flexible CG's warm start: r = b - A x, level-0 right-hand side (f32: float) = r, the
+// level-0 iterate zeroed when zero_x0, and FIN_FCG_SETUP_WARM's <b,b>, <r,r>
+int warm_residual(hs_ctx* c, const double* b, const double* x, double* r, void* b0, void* x0, bool zero_x0,
+                  bool f32);
 int fcg_spmv_dot(hs_ctx* c, const void* z, bool f32, const void* p, void* pn, double* q);
 int mask_fluid(hs_ctx* c, double* x);
 int true_relres(hs_ctx* c, const double* b, const double* x, double* relres, double* bnorm);

@@ -989,6 +989,43 @@ __global__ __launch_bounds__(BT) void k_resid_norms(OpArgs o, const double* x, c
     hsd::put_partial2<BT>(rr, bb, tid, blockIdx.x, gx * gy * gz, partials);
 }
 
+// Residual of a nonzero initial iterate x (flexible CG's warm start) in one pass: r = b - A x
+// on every interior cell (b itself off the fluid cells, where b and the rows of A are zero),
+// the level-0 right-hand side (T) r, the level-0 iterate zeroed when zero_x0, and block
+// partials of <b,b> then <r,r> (two per block).  Padding entries of r, b0 and x0 are not
+// written: they hold zeros from allocation and no kernel writes them.
+template <typename T>
+__global__ __launch_bounds__(BT) void k_warm_resid(OpArgs o, const double* x, const double* b, double* r, T* b0,
+                                                   T* x0, int zero_x0, int kc, int gx, int gy, int gz,
+                                                   double* partials) {
+    const Dims& d = o.d;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const hsd::Tile tl = hsd::tile_of(gx, gy, gz);
+    const long i = 1 + tl.bx * BX + threadIdx.x, j = 1 + tl.by * BY + threadIdx.y;
+    double rr = 0.0, bb = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        const int k0 = 1 + tl.bz * kc, k1 = min(d.nz, k0 + kc - 1);
+        spmv_column(
+            o, [x](long q) { return x[q]; }, i, j, k0, k1,
+            [&](long c, double y, double) {
+                const double bc = b[c], rv = bc - y;
+                r[c] = rv;
+                b0[c] = (T)rv;
+                rr += rv * rv;
+                bb += bc * bc;
+            },
+            [&](long c, double) {
+                if (zero_x0) x0[c] = T(0);
+                if (o.cls[c] != CELL_FLUID) {
+                    const double bc = b[c];
+                    r[c] = bc;
+                    b0[c] = (T)bc;
+                }
+            });
+    }
+    hsd::put_partial2<BT>(bb, rr, tid, blockIdx.x, gx * gy * gz, partials);
+}
+
 // zero entries outside fluid cells
 __global__ void k_mask_fluid(long n, const uint8_t* cls, double* x) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
@@ -1476,6 +1513,23 @@ int cg_solve(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resi
     }
     if (h->nan || !std::isfinite(h->rr)) return HS_E_NAN;
     return HS_OK;
+}
+
+int warm_residual(hs_ctx* c, const double* b, const double* x, double* r, void* b0, void* x0, bool zero_x0,
+                  bool f32) {
+    const Dims& d = c->d;
+    const long tiles = (long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+    const int kc = kchunk(d.nz, tiles);
+    const dim3 gr = grid3(d.nx, d.ny, d.nz, kc);
+    const int nb = (int)(gr.x * gr.y * gr.z);
+    if (f32)
+        k_warm_resid<float><<<nb, dim3(BX, BY), 0, c->stream>>>(op_args(c), x, b, r, (float*)b0, (float*)x0,
+                                                                 zero_x0 ? 1 : 0, kc, gr.x, gr.y, gr.z, c->partials);
+    else
+        k_warm_resid<double><<<nb, dim3(BX, BY), 0, c->stream>>>(op_args(c), x, b, r, (double*)b0, (double*)x0,
+                                                                  zero_x0 ? 1 : 0, kc, gr.x, gr.y, gr.z, c->partials);
+    HS_CHECK(hipGetLastError());
+    return launch_finish(c, FIN_FCG_SETUP_WARM, nb, nullptr);
 }
 
 int launch_resid_norms(hs_ctx* c, const double* b, const double* x) {

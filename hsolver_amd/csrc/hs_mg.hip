@@ -661,24 +661,6 @@ __global__ __launch_bounds__(FLAT) void k_fcg_setup(long n, const double* b, dou
     hsd::put_partial<FLAT>(acc, threadIdx.x, blockIdx.x, partials);
 }
 
-// the setup from a nonzero iterate x (left in place) with q = A x: r = b - q; partials
-// of <b,b> and <r,r>
-template <typename T>
-__global__ __launch_bounds__(FLAT) void k_fcg_setup_warm(long n, const double* b, const double* q, double* r, T* b0,
-                                                         T* x0, int zero_x0, double* partials) {
-    double bb = 0.0, rr = 0.0;
-    GRID_STRIDE(t, n) {
-        const double bv = b[t];
-        const double rv = bv - q[t];
-        r[t] = rv;
-        b0[t] = (T)rv;
-        if (zero_x0) x0[t] = T(0);
-        bb += bv * bv;
-        rr += rv * rv;
-    }
-    hsd::put_partial2<FLAT>(bb, rr, threadIdx.x, blockIdx.x, gridDim.x, partials);
-}
-
 // the zero start after a warm one that was no better (ctl->cold): x = 0, r = b
 template <typename T>
 __global__ __launch_bounds__(FLAT) void k_fcg_cold(long n, const double* b, double* x, double* r, T* b0, T* x0,
@@ -998,9 +980,7 @@ int fcg(hs_ctx* c, const double* b, double* x, hs_stats* st, bool with_resid) {
             c->khist[0] = c->kx;
             c->kx = x = next;
         }
-        HS_TRY(spmv(c, x, q));
-        k_fcg_setup_warm<T><<<G, FLAT, 0, c->stream>>>(n, b, q, r, b0, z(), zero_x0, c->partials);
-        HS_TRY(launch_finish(c, FIN_FCG_SETUP_WARM, G, nullptr));
+        HS_TRY(warm_residual(c, b, x, r, b0, z(), zero_x0 != 0, sizeof(T) == sizeof(float)));
         k_fcg_cold<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->sctl);
     } else {
         k_fcg_setup<T><<<G, FLAT, 0, c->stream>>>(n, b, x, r, b0, z(), zero_x0, c->partials);
